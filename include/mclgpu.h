@@ -1,0 +1,175 @@
+/*
+ * libmclgpu -- MI355X (gfx950) checkerboard Metropolis engine for McLics-style nematic lattices. C ABI.
+ *
+ * The library replaces the GPU sweep engine of McLics behind its mode loops: the EvolveNGPU class
+ * (include/evolve.cuh:25-28 of the reference) with Monte_Carlo_Step_GPU (src/evolve.cu:317-440),
+ * energy_calculator_GPU (src/evolve.cu:442-478), the host Q_tensor / Eigen_value_evaluation calls of the mode
+ * loops (src/evolve_thermal.cu:85-96, src/parameter_order.cpp:19-71) and the device physics of src/potential.cu,
+ * src/geometry.cu. The mode loops (thermalGPU, stepGPU, quenchGPU, electricGPU) stay in the host program and call
+ * the entry points below.
+ *
+ * Conventions (the reference's, include/define.h:9-19):
+ *   - site (i, j, k) has the linear index idx = i + nx * (j + ny * k) (x fastest);
+ *   - directors are passed as AoS float[3 * nx * ny * nz]; the library stores them in its own layout on the device;
+ *   - point types: 0 empty, 1 bulk, n >= 2 site of surface n - 2 (int32, as the reference's pt array);
+ *   - normals: AoS float[3 * Nt], the outward unit normal on surface sites (Geometry::ns, include/geometry.h:274).
+ * Ownership: the caller owns every host buffer; the context owns all device memory (released by mcl_destroy).
+ * Errors: every call returns MCL_OK or an error code; mcl_last_error() gives the message of the calling thread's
+ * last failure. The library never calls exit(); the host maps MCL_ERR_DEVICE to the reference's exit(20)
+ * (src/evolve.cu:325-475).
+ * Threading: a context is used by one host thread at a time; several contexts may coexist in a process.
+ *
+ * A context holds one or more independent lattices ("replicas") that are swept together in the same kernel
+ * launches; a single simulation is a context with one lattice.
+ */
+#ifndef MCLGPU_H_
+#define MCLGPU_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MCL_ABI_VERSION 1
+
+enum mcl_status {
+  MCL_OK = 0,
+  MCL_ERR_ARG = 1,      /* invalid argument (sizes, odd periodic axis, unknown type, ...) */
+  MCL_ERR_NOMEM = 2,    /* device or host allocation failed */
+  MCL_ERR_DEVICE = 20   /* HIP runtime or kernel error (the reference exits with status 20) */
+};
+
+/* Pair potentials: Bulk_Energy_Lebwohl_Lasher / _GHRL / _Selinger_Pear (src/potential.cpp:19-56). */
+enum mcl_potential { MCL_POT_LL = 0, MCL_POT_GHRL = 1, MCL_POT_PEAR = 2 };
+
+/* Anchoring types, in the order of include/anchoring.h:29-104 (input names in parentheses). */
+enum mcl_anchoring {
+  MCL_ANCH_HOMEOTROPIC = 0,      /* homeotropic      -W (n.s)^2 */
+  MCL_ANCH_RP = 1,               /* rp               -W (n.e)^2, e = axis */
+  MCL_ANCH_RP_TANGENT = 2,       /* rp_tangent       -W (n.e)^2, e = axis projected on the tangent plane */
+  MCL_ANCH_FG = 3,               /* fg               +W (n.s)^2 */
+  MCL_ANCH_HOMEOTROPIC_GHRL = 4, /* homeotropic_ghrl  W U_GHRL(n, s, r = s) */
+  MCL_ANCH_RP_GHRL = 5,          /* rp_ghrl           W U_GHRL(n, axis, r = s) */
+  MCL_ANCH_FG_GHRL = 6           /* fg_ghrl           W U_GHRL(n, tangent projection of n, r = s) */
+};
+
+/* Radial observables (parameter_order.cpp:223-245), selected by the `radial` input (main.cpp:90-96). */
+enum mcl_radial { MCL_RADIAL_NONE = 0, MCL_RADIAL_SPHERE = 1, MCL_RADIAL_CYLINDER = 2, MCL_RADIAL_BOTH = 3 };
+
+/* One surface: its anchoring type, strength W and lab easy axis (Anchoring::lab_axis of theta_s, phi_s,
+ * anchoring.cpp:19-24; unused by homeotropic, fg, homeotropic_ghrl and fg_ghrl). */
+typedef struct mcl_surface {
+  int32_t type;
+  float W;
+  float axis[3];
+} mcl_surface;
+
+/* Physics of one lattice: the fields of Parameters (include/parameters.h:39-96) that the energy reads.
+ * potential and flip_move must be the same for every lattice of a context. */
+typedef struct mcl_model {
+  int32_t potential;
+  float A, B1, C;                                           /* LL and pear couplings */
+  float ghrl_lambda, ghrl_mu, ghrl_nu, ghrl_rho, ghrl_sigma; /* from setGHRL (src/io.cpp:346-371) */
+  float elec_dir[3];                                        /* unit field direction (normalized, main.cpp:97-106) */
+  float elecA, elecP;                                       /* dielectric and dipolar couplings */
+  float elecE;                                              /* field magnitude (1 outside the electric mode) */
+  int32_t flip_move;                                        /* n -> -n trial (main.cpp:88) */
+} mcl_model;
+
+/* One lattice. */
+typedef struct mcl_lattice_desc {
+  int32_t nx, ny, nz;
+  int32_t periodic[3];       /* 1 periodic, 0 free; periodic axes must have even sizes (io.cpp:462-475) */
+  const float *ni;           /* initial directors, AoS */
+  const int32_t *pt;         /* point types, or NULL for an all-bulk lattice */
+  const float *ns;           /* outward normals, AoS; may be NULL when nsurf == 0 */
+  int32_t nsurf;
+  const mcl_surface *surf;   /* nsurf entries (surface n = pt - 2) */
+  mcl_model model;
+  float T;                   /* temperature */
+  float ang_var;             /* initial maximum rotation (0.5 in the reference, evolve_thermal.cu:43) */
+  uint64_t seed;             /* RNG key: independent streams for different (seed, lattice index) */
+  int32_t radial;            /* enum mcl_radial */
+} mcl_lattice_desc;
+
+/* Running sums over the accumulated measurements of one lattice (evolve_thermal.cu:84-101 keeps S1, S2, E, E2
+ * and the ExtraObservables sums on the host; here they are FP64 sums on the device). */
+typedef struct mcl_moments {
+  int64_t n;                       /* number of measurements accumulated */
+  double S, S2, E, E2;             /* sums of S, S^2, E, E^2 (E = mean site energy) */
+  double Sr_sph, Pr_sph, Sr_cyl, Pr_cyl; /* sums of the radial observables */
+  double P, dtheta, divn;          /* sums of the pear observables (parameter_order.cpp:131-215) */
+} mcl_moments;
+
+/* One measurement of the current state. */
+typedef struct mcl_observables {
+  int64_t nvalid;                  /* non-empty sites */
+  double E;                        /* mean site energy (energy_calculator_GPU) */
+  double Q[6];                     /* Q tensor xx, xy, xz, yy, yz, zz (Q_tensor) */
+  double S;                        /* largest eigenvalue of Q (Eigen_value_evaluation) */
+  double director[3];              /* its eigenvector */
+  double Sr_sph, Pr_sph, Sr_cyl, Pr_cyl;
+  double P, dtheta, divn;
+} mcl_observables;
+
+/* Creates a context with nlat lattices on the current HIP device. Replaces the device setup of the mode loops
+ * (evolve_thermal.cu:48-66) and the first-call setup of Monte_Carlo_Step_GPU (evolve.cu:329-406). */
+int mcl_create(const mcl_lattice_desc *lat, int32_t nlat, void **ctx_out);
+void mcl_destroy(void *ctx);
+
+/* Runs nsweeps sweeps of every lattice (Monte_Carlo_Step_GPU, evolve.cu:317-440: one trial rotation per non-empty
+ * site, Metropolis acceptance, optional flip trial, ang_var adaptation after each sweep). When measure_every > 0,
+ * a measurement (energy_calculator_GPU + Q_tensor + Eigen_value_evaluation + ExtraObservables) is accumulated into
+ * the moments after every measure_every-th sweep. Asynchronous: returns once the work is queued. */
+int mcl_sweeps(void *ctx, int32_t nsweeps, int32_t measure_every);
+
+/* Sets the temperature (d_T, evolve_thermal.cu:79-80) or the field magnitude (evolve_electric.cu:306-307) of one
+ * lattice, or of all lattices when lat < 0. */
+int mcl_set_temperature(void *ctx, int32_t lat, float T);
+int mcl_set_field(void *ctx, int32_t lat, float elecE);
+int mcl_set_ang_var(void *ctx, int32_t lat, float ang_var);
+int mcl_get_ang_var(void *ctx, int32_t lat, float *ang_var);
+/* Accepted rotations of the last completed sweep and the number of completed sweeps (synchronizes). */
+int mcl_get_sweep_stats(void *ctx, int32_t lat, int64_t *last_accepted, int64_t *sweeps_done);
+
+/* Moments: reset (S1 = S2 = E = E2 = 0, evolve_thermal.cu:84) and read (synchronizes). */
+int mcl_reset_moments(void *ctx);
+int mcl_get_moments(void *ctx, int32_t lat, mcl_moments *out);
+/* Measures the current state now without accumulating it (synchronizes). */
+int mcl_measure(void *ctx, int32_t lat, mcl_observables *out);
+
+/* Director transfer in the AoS host layout (the cudaMemcpy of d_ni, evolve_thermal.cu:90 and :99). */
+int mcl_get_directors(void *ctx, int32_t lat, float *ni);
+int mcl_set_directors(void *ctx, int32_t lat, const float *ni);
+/* Local order parameter of every site for the snapshots (lattice_order_parameter, parameter_order.cpp:75-127):
+ * out[idx] = largest eigenvalue of Q over the site and its non-empty neighbours; NaN on empty sites. */
+int mcl_local_order(void *ctx, int32_t lat, float *out);
+
+int mcl_synchronize(void *ctx);
+const char *mcl_last_error(void);
+int mcl_abi_version(void);
+
+/* ---- verification entry points (no reference counterpart: they expose the device functions to parity tests) ---- */
+
+/* Site energies computed by the sweep kernel's own device energy function: for each k, e_cur[k] is the energy of
+ * site sites[k] with its current director and e_trial[k] with director trial[3k..3k+2] (the quantities compared
+ * in MC_GPU, evolve.cu:205-209). */
+int mcl_site_energies(void *ctx, int32_t lat, const int32_t *sites, const float *trial, int32_t n, float *e_cur,
+                      float *e_trial);
+/* Philox4x32-10 of the device generator: out[4k..4k+3] = philox(ctr[4k..4k+3], key[2k..2k+1]). */
+int mcl_philox(const uint32_t *ctr, const uint32_t *key, int32_t n, uint32_t *out);
+
+/* ---- timing (used by the benchmark) ---- */
+/* When stride > 0, mcl_sweeps records HIP events around each of the two sweep-kernel launches of every
+ * stride-th sweep (events for max_timed_sweeps timed sweeps are created up front); mcl_kernel_timing returns the
+ * number of timed launches and their summed duration (synchronizes). stride = 0 switches timing off. */
+int mcl_set_kernel_timing(void *ctx, int32_t stride, int32_t max_timed_sweeps);
+int mcl_kernel_timing(void *ctx, int64_t *launches, double *total_ms);
+/* Device stream of the context (a hipStream_t), for callers that record their own events. */
+int mcl_get_stream(void *ctx, void **stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

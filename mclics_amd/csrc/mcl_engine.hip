@@ -1,0 +1,1500 @@
+// libmclgpu: checkerboard Metropolis sweep, FP64 observables and their C ABI, written for MI355X (gfx950).
+//
+// Device layout (per lattice, pooled over the lattices of a context):
+//   The lattice is split into its two checkerboard colours c = (i + j + k) & 1. Site (i, j, k) of colour c lives at
+//   position t = (i >> 1) + nxh * (j + ny * k) of that colour's block (nxh = ceil(nx / 2)), in three FP32 arrays
+//   X, Y, Z (structure of arrays). All neighbours of a site have the other colour, and the six neighbours of the
+//   sites t, t+1, ... of one row are consecutive entries of rows of the other colour, so every load of a sweep
+//   pass is a coalesced 4-byte-per-lane access. Point types (uint8) and the per-site anchoring vector follow the
+//   same layout and exist only when a lattice has empty or surface sites.
+//
+// One sweep = two kernel launches (colour 0, then colour 1). Each thread makes one Metropolis trial on one site of
+// the active colour (proposal of evolve.cu:162-182, energies of geometry.cu:25-120 / potential.cu:11-58, rule of
+// evolve.cu:212-239). Random numbers come from Philox4x32-10 keyed by (seed, lattice) with counter
+// (site, sweep), so the generator has no state in memory. Accepted rotations are counted with a wave ballot and
+// stored per workgroup; a small close kernel (one workgroup per lattice) sums them, applies the ang_var rule of
+// evolve.cu:430-436 on the device and advances the sweep counter, so no host round trip is needed per sweep.
+//
+// A measurement is one pass over both colours that accumulates, in FP64, the site energy (get_Energy,
+// evolve.cu:66-108), the Q-tensor sums (parameter_order.cpp:40-71), the radial order (parameter_order.cpp:223-245)
+// and the pear observables (parameter_order.cpp:131-215) into one partial per workgroup; a finalize kernel sums the
+// partials of each lattice in a fixed order (bitwise reproducible), solves the 3x3 eigenproblem for S in FP64 and
+// adds the sample to the lattice's FP64 moments.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstddef>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/mclgpu.h"
+#include "philox.h"
+
+namespace {
+
+constexpr int kBlock = 256;          // threads per workgroup (4 wave64)
+constexpr int kWaves = kBlock / 64;
+constexpr int kMeasPerThread = 4;    // sites per thread in a measurement workgroup
+constexpr int kNQ = 24;              // FP64 quantities per measurement partial
+constexpr uint32_t kTagSweep = 0x4D435357u;  // counter word 3 of the sweep draws
+
+// Lattice flags.
+constexpr int F_PX = 1, F_PY = 2, F_PZ = 4, F_MASK = 8, F_SURF = 16, F_RSPH = 32, F_RCYL = 64;
+
+// Division by a runtime constant d for 0 <= n < 2^31: q = (n * m) >> sh with l = ceil(log2 d),
+// m = ceil(2^(31+l) / d), sh = 31 + l (Granlund & Montgomery 1994; Hacker's Delight 10-8: exact because
+// m d - 2^(31+l) < d <= 2^l).
+struct FastDiv {
+  uint32_t m, sh;
+};
+FastDiv make_fastdiv(uint32_t d) {
+  uint32_t l = 0;
+  while ((1ull << l) < d) l++;
+  uint64_t m = ((1ull << (31 + l)) + d - 1) / d;
+  return FastDiv{(uint32_t)m, 31 + l};
+}
+__host__ __device__ __forceinline__ uint32_t fdiv(uint32_t n, uint32_t m, uint32_t sh) {
+  return (uint32_t)(((uint64_t)n * m) >> sh);
+}
+
+struct LatDev {
+  int nx, ny, nz, nxh, nc;  // nc = sites per colour block = nxh * ny * nz
+  uint32_t mxh, sxh, my, sy;
+  int flags;
+  int nvalid;
+  uint32_t off;   // pooled offset of the colour-0 block; colour 1 follows at off + nc (pool < 2^31 sites)
+  uint32_t key0, key1;
+  int nblk;       // sweep workgroups per colour
+  int blk0;       // first sweep workgroup of this lattice in the work list
+  int surf0;      // first entry of this lattice in the pooled surface table
+  int mp0, mp1;   // measurement partials [mp0, mp1)
+  float A, B1, C, gl, gm, gn, gr, gs;
+  float ex, ey, ez, elecA, elecP;
+  float cx, cy, cz;  // centre of the radial observables (Nx/2, Ny/2, Nz/2 with integer division)
+};
+
+struct CtrlDev {
+  float T, elecE, ang_var;
+  unsigned int last_acc;  // accepted rotations of the last completed sweep
+  unsigned long long sweep;
+};
+
+// Device anchoring kernels (the GPU selection of evolve.cu:280-297): 0 = -W (n.v)^2 (homeotropic, rp, rp_tangent,
+// with v the per-site vector), 1 = +W (n.v)^2 (fg), 2 = W U_GHRL(n, v, v), 3 = W U_GHRL(n, axis, v),
+// 4 = W U_GHRL(n, tangent part of n, v).
+struct SurfDev {
+  int kind;
+  float W, ax, ay, az;
+};
+
+struct MomDev {
+  double n, S, S2, E, E2, srs, prs, src, prc, P, dth, divn;
+};
+struct ObsDev {
+  double nvalid, E, Q[6], S, dir[3], srs, prs, src, prc, P, dth, divn;
+};
+
+struct V3 {
+  float x, y, z;
+};
+__device__ __forceinline__ float dot3(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+template <int AX>
+__device__ __forceinline__ float comp(V3 a) {
+  return AX == 0 ? a.x : (AX == 1 ? a.y : a.z);
+}
+
+// ---- pair potentials (potential.cpp:19-56 / potential.cu:11-43) ----
+
+// GHRL with a general bond vector r (used by the GHRL anchorings, geometry.cu:377-390).
+__device__ __forceinline__ float ghrl_general(const LatDev &L, V3 ni, V3 nj, V3 r) {
+  const float ai = dot3(ni, r), aj = dot3(nj, r), nij = dot3(ni, nj);
+  const float pij = 1.5f * nij * nij - 0.5f;
+  const float cross =
+      (ni.z * nj.y - ni.y * nj.z) * r.x + (ni.x * nj.z - ni.z * nj.x) * r.y + (ni.y * nj.x - ni.x * nj.y) * r.z;
+  const float E1 = 1.5f * ai * ai + 1.5f * aj * aj - 1.0f;
+  return E1 * (L.gr * pij + L.gl) + L.gm * (ai * aj * nij - (1.0f / 9.0f)) + L.gn * pij +
+         L.gs * (nij > 0.0f ? cross : -cross);
+}
+
+// Bond energy with the neighbour in direction D (0 +x, 1 -x, 2 +y, 3 -y, 4 +z, 5 -z); r = +-e_axis is a
+// compile-time constant, so a_i, a_j and the cross product reduce to single components. For LL the caller
+// multiplies the sum of nij^2 by -1.5 A.
+template <int POT, int D>
+__device__ __forceinline__ float bond(const LatDev &L, V3 a, V3 b) {
+  constexpr int AX = D >> 1;
+  constexpr float SG = (D & 1) ? -1.0f : 1.0f;
+  const float nij = dot3(a, b);
+  if constexpr (POT == MCL_POT_LL) {
+    return nij * nij;
+  } else if constexpr (POT == MCL_POT_GHRL) {
+    const float ai = SG * comp<AX>(a), aj = SG * comp<AX>(b);
+    const float pij = 1.5f * nij * nij - 0.5f;
+    // c = r.(n_j x n_i) (potential.cpp:50-52)
+    float cr = AX == 0 ? (a.z * b.y - a.y * b.z) : (AX == 1 ? (a.x * b.z - a.z * b.x) : (a.y * b.x - a.x * b.y));
+    cr *= SG;
+    const float E1 = 1.5f * (ai * ai + aj * aj) - 1.0f;
+    return E1 * (L.gr * pij + L.gl) + L.gm * (ai * aj * nij - (1.0f / 9.0f)) + L.gn * pij +
+           L.gs * (nij > 0.0f ? cr : -cr);
+  } else {
+    // pear: -A b^2 - B1 b - (C/4) (1 + b)^2 r.(n_j - n_i)
+    const float dr = SG * (comp<AX>(b) - comp<AX>(a));
+    const float op = 1.0f + nij;
+    return -L.A * nij * nij - L.B1 * nij - (0.25f * L.C) * (op * op) * dr;
+  }
+}
+
+__device__ __forceinline__ float surface_energy(const LatDev &L, const SurfDev &s, V3 n, V3 v) {
+  switch (s.kind) {
+    case 0: {
+      const float d = dot3(n, v);
+      return -s.W * d * d;
+    }
+    case 1: {
+      const float d = dot3(n, v);
+      return s.W * d * d;
+    }
+    case 2:
+      return s.W * ghrl_general(L, n, v, v);
+    case 3:
+      return s.W * ghrl_general(L, n, V3{s.ax, s.ay, s.az}, v);
+    default: {
+      const float d = dot3(n, v);
+      V3 t{n.x - d * v.x, n.y - d * v.y, n.z - d * v.z};
+      const float mod = sqrtf(dot3(t, t));
+      if (mod < 1e-6f) return 0.0f;
+      t.x /= mod;
+      t.y /= mod;
+      t.z /= mod;
+      return s.W * ghrl_general(L, n, t, v);
+    }
+  }
+}
+
+// Local environment of one site (the nni fullni[8] of define.h:24-33, in registers).
+struct Site {
+  V3 n;
+  V3 nb[6];
+  V3 sv;
+  unsigned ex;  // bit d: neighbour d exists and is not empty
+  int ptv;
+  int i, j, k;
+};
+
+// Site energy: bonds + anchoring + field (Bulk/Slab/Sphere/Custom_site_energy_GPU, geometry.cu:63-120).
+template <int POT, bool SURF, bool FIELD>
+__device__ __forceinline__ float site_energy(const LatDev &L, const Site &s, V3 n, const SurfDev *__restrict__ surf,
+                                             float elecE) {
+  float e = 0.0f;
+  e += (s.ex & 1u) ? bond<POT, 0>(L, n, s.nb[0]) : 0.0f;
+  e += (s.ex & 2u) ? bond<POT, 1>(L, n, s.nb[1]) : 0.0f;
+  e += (s.ex & 4u) ? bond<POT, 2>(L, n, s.nb[2]) : 0.0f;
+  e += (s.ex & 8u) ? bond<POT, 3>(L, n, s.nb[3]) : 0.0f;
+  e += (s.ex & 16u) ? bond<POT, 4>(L, n, s.nb[4]) : 0.0f;
+  e += (s.ex & 32u) ? bond<POT, 5>(L, n, s.nb[5]) : 0.0f;
+  if constexpr (POT == MCL_POT_LL) e *= -1.5f * L.A;
+  if constexpr (SURF) {
+    if (s.ptv > 1) e += surface_energy(L, surf[L.surf0 + s.ptv - 2], n, s.sv);
+  }
+  if constexpr (FIELD) {
+    const float nE = elecE * (n.x * L.ex + n.y * L.ey + n.z * L.ez);
+    e += -L.elecP * nE - L.elecA * nE * nE;
+  }
+  return e;
+}
+
+// Decodes colour-block position t of lattice L into (i, j, k) and loads the site and its neighbours.
+// Returns false for padding positions (i >= nx) and empty sites.
+template <bool MASK, bool SURF>
+__device__ __forceinline__ bool gather(const LatDev &L, int colour, int t, const float *__restrict__ X,
+                                       const float *__restrict__ Y, const float *__restrict__ Z,
+                                       const uint8_t *__restrict__ PT, const float *__restrict__ SX,
+                                       const float *__restrict__ SY, const float *__restrict__ SZ, Site &s,
+                                       uint32_t &self) {
+  const int nx = L.nx, ny = L.ny, nz = L.nz, nxh = L.nxh;
+  const uint32_t row = fdiv((uint32_t)t, L.mxh, L.sxh);
+  const int m = t - (int)row * nxh;
+  const int k = (int)fdiv(row, L.my, L.sy);
+  const int j = (int)row - k * ny;
+  if (m < 0 || m >= nxh || j < 0 || j >= ny || k >= nz) return false;  // unreachable for t < nc
+  const int i = 2 * m + ((j + k + colour) & 1);
+  if (i >= nx) return false;
+  s.i = i;
+  s.j = j;
+  s.k = k;
+  const uint32_t base = L.off + (uint32_t)(colour * L.nc);
+  const uint32_t obase = L.off + (uint32_t)((1 - colour) * L.nc);
+  self = base + (uint32_t)t;
+  s.ptv = 1;
+  if constexpr (MASK) {
+    s.ptv = PT[self];
+    if (s.ptv == 0) return false;
+  }
+  s.n = V3{X[self], Y[self], Z[self]};
+  // Periodic_Boundary / Free_Boundary (potential.cu:50-58)
+  const bool px = L.flags & F_PX, py = L.flags & F_PY, pz = L.flags & F_PZ;
+  int ip = i + 1, im = i - 1, jp = j + 1, jm = j - 1, kp = k + 1, km = k - 1;
+  unsigned ex = 63u;
+  if (ip == nx) {
+    if (px) ip = 0; else ex &= ~1u;
+  }
+  if (im < 0) {
+    if (px) im = nx - 1; else ex &= ~2u;
+  }
+  if (jp == ny) {
+    if (py) jp = 0; else ex &= ~4u;
+  }
+  if (jm < 0) {
+    if (py) jm = ny - 1; else ex &= ~8u;
+  }
+  if (kp == nz) {
+    if (pz) kp = 0; else ex &= ~16u;
+  }
+  if (km < 0) {
+    if (pz) km = nz - 1; else ex &= ~32u;
+  }
+  const uint32_t rowoff = obase + (uint32_t)(nxh * (j + ny * k));
+  uint32_t nidx[6];
+  nidx[0] = rowoff + (uint32_t)(ip >> 1);
+  nidx[1] = rowoff + (uint32_t)(im >> 1);
+  nidx[2] = obase + (uint32_t)(m + nxh * (jp + ny * k));
+  nidx[3] = obase + (uint32_t)(m + nxh * (jm + ny * k));
+  nidx[4] = obase + (uint32_t)(m + nxh * (j + ny * kp));
+  nidx[5] = obase + (uint32_t)(m + nxh * (j + ny * km));
+#pragma unroll
+  for (int d = 0; d < 6; d++) {
+    if (ex & (1u << d)) {
+      const uint32_t q = nidx[d];
+      s.nb[d] = V3{X[q], Y[q], Z[q]};
+      if constexpr (MASK) {
+        if (PT[q] == 0) ex &= ~(1u << d);
+      }
+    } else {
+      s.nb[d] = V3{0.0f, 0.0f, 1.0f};
+    }
+  }
+  s.ex = ex;
+  s.sv = V3{0.0f, 0.0f, 0.0f};
+  if constexpr (SURF) {
+    if (s.ptv > 1) s.sv = V3{SX[self], SY[self], SZ[self]};
+  }
+  return true;
+}
+
+// ang_var rule of evolve.cu:430-436 (float ang_var, double comparisons as in the reference).
+__device__ __forceinline__ float ang_var_rule(float av, unsigned acc, int nvalid) {
+  if (nvalid <= 0) return av;
+  const double frac = (double)acc / (double)nvalid;
+  if (frac < 0.5) {
+    av = (float)((double)av * 0.99);
+    if ((double)av < 0.01) av = 0.01f;
+  } else if (frac > 0.5) {
+    av = (float)((double)av / 0.99);
+    if ((double)av > 1.0) av = (float)((double)av - 0.5);
+  }
+  return av;
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// Sweep pass over one colour of every lattice. blk[L] = (lattice, first colour-block position) of logical
+// workgroup L; consecutive logical workgroups are placed on the same XCD (workgroups are dispatched to the 8 XCDs
+// round robin), so the rows a workgroup shares with its neighbours in z and y mostly hit the same L2.
+template <int POT, bool MASK, bool SURF, bool FIELD, bool FLIP>
+__global__ __launch_bounds__(kBlock) void k_sweep(const LatDev *__restrict__ lats, const CtrlDev *__restrict__ ctrl,
+                                                  unsigned int *__restrict__ bcount, const int2 *__restrict__ blk,
+                                                  int nblk, int chunk, int colour,
+                                                  float *__restrict__ X, float *__restrict__ Y,
+                                                  float *__restrict__ Z, const uint8_t *__restrict__ PT,
+                                                  const float *__restrict__ SX, const float *__restrict__ SY,
+                                                  const float *__restrict__ SZ, const SurfDev *__restrict__ surf) {
+  const int b = blockIdx.x;
+  const int Lb = (b & 7) * chunk + (b >> 3);
+  if (Lb >= nblk) return;
+  const int2 bd = blk[Lb];
+  const int r = bd.x;
+  const LatDev &L = lats[r];
+  const float T = ctrl[r].T, av = ctrl[r].ang_var, elecE = ctrl[r].elecE;
+  const unsigned long long sweep = ctrl[r].sweep;
+  const int t = bd.y + (int)threadIdx.x;
+
+  bool accepted = false;
+  Site s;
+  uint32_t self = 0;
+  if (t < L.nc && gather<MASK, SURF>(L, colour, t, X, Y, Z, PT, SX, SY, SZ, s, self)) {
+    const uint32_t lin = (uint32_t)(s.i + L.nx * (s.j + L.ny * s.k));
+    const mcl::Philox4 R =
+        mcl::philox4x32_10(lin, (uint32_t)sweep, (uint32_t)(sweep >> 32), kTagSweep, L.key0, L.key1);
+    // trial rotation by pi * va about x, y or z (evolve.cu:162-182)
+    const float va = (2.0f * mcl::u01_open(R.v[0]) - 1.0f) * av;
+    const float rot = mcl::u01_open(R.v[1]);
+    float sn, cs;
+    sincospif(va, &sn, &cs);
+    const V3 n = s.n;
+    V3 nn;
+    if (rot < 0.333f) {
+      nn = V3{n.x, n.y * cs + n.z * sn, n.z * cs - n.y * sn};
+    } else if (rot < 0.666f) {
+      nn = V3{n.x * cs - n.z * sn, n.y, n.z * cs + n.x * sn};
+    } else {
+      nn = V3{n.x * cs + n.y * sn, n.y * cs - n.x * sn, n.z};
+    }
+    const float inv = 1.0f / sqrtf(dot3(nn, nn));
+    nn.x *= inv;
+    nn.y *= inv;
+    nn.z *= inv;
+    const float E_old = site_energy<POT, SURF, FIELD>(L, s, n, surf, elecE);
+    const float E_new = site_energy<POT, SURF, FIELD>(L, s, nn, surf, elecE);
+    const float u = mcl::u01_open(R.v[2]);
+    // Metropolis; T = 0 accepts when the energy does not increase (evolve.cu:212-224)
+    accepted = T > 0.0f ? (u <= __expf(-(E_new - E_old) / T)) : (E_new <= E_old);
+    V3 nc = accepted ? nn : n;
+    bool changed = accepted;
+    if constexpr (FLIP) {
+      // separate n -> -n trial, not counted in the acceptance (evolve.cu:225-239)
+      const float E_cur = accepted ? E_new : E_old;
+      const V3 nf{-nc.x, -nc.y, -nc.z};
+      const float E_f = site_energy<POT, SURF, FIELD>(L, s, nf, surf, elecE);
+      const float uf = mcl::u01_open(R.v[3]);
+      if (T > 0.0f ? (uf <= __expf(-(E_f - E_cur) / T)) : (E_f <= E_cur)) {
+        nc = nf;
+        changed = true;
+      }
+    }
+    if (changed) {
+      X[self] = nc.x;
+      Y[self] = nc.y;
+      Z[self] = nc.z;
+    }
+  }
+
+  // accepted rotations: wave ballot, then one plain store per workgroup into its own slot (no atomics: same-address
+  // device-scope atomics from thousands of workgroups on 8 XCDs serialize)
+  __shared__ unsigned int wsum[kWaves];
+  const unsigned long long ball = __ballot(accepted ? 1 : 0);
+  if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = (unsigned)__popcll(ball);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned tot = 0;
+#pragma unroll
+    for (int w = 0; w < kWaves; w++) tot += wsum[w];
+    bcount[colour * nblk + Lb] = tot;
+  }
+}
+
+// Closes a sweep (one workgroup per lattice): sums the accepted counts of the lattice's workgroups of both colour
+// passes in a fixed order, applies the ang_var rule of evolve.cu:430-436 and advances the sweep counter.
+__global__ __launch_bounds__(kBlock) void k_close_sweep(const LatDev *__restrict__ lats, CtrlDev *__restrict__ ctrl,
+                                                        const unsigned int *__restrict__ bcount, int nblk) {
+  const int r = blockIdx.x;
+  const LatDev &L = lats[r];
+  unsigned v = 0;
+  for (int b = (int)threadIdx.x; b < L.nblk; b += kBlock) v += bcount[L.blk0 + b] + bcount[nblk + L.blk0 + b];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  __shared__ unsigned int ws[kWaves];
+  if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned a = 0;
+    for (int w = 0; w < kWaves; w++) a += ws[w];
+    CtrlDev c = ctrl[r];
+    c.ang_var = ang_var_rule(c.ang_var, a, L.nvalid);
+    c.last_acc = a;
+    c.sweep += 1ull;
+    ctrl[r] = c;
+  }
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// Measurement: per-workgroup FP64 partial sums over kBlock * kMeasPerThread sites of one colour of one lattice.
+// Quantities: 0 E, 1..5 sum n_a n_b (xx xy xz yy yz), 6 nvalid, 7..9 radial sphere (sum Sr, sum c, count),
+// 10..12 radial cylinder, 13..15 sum n, 16..18 splay vector, 19 bonds, 20 sum div n, 21 sum n_z^2.
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+template <int POT, bool MASK, bool SURF, bool FIELD, bool PEAR>
+__global__ __launch_bounds__(kBlock) void k_measure(const LatDev *__restrict__ lats, const CtrlDev *__restrict__ ctrl,
+                                                    const int4 *__restrict__ mblk, int nmblk,
+                                                    const float *__restrict__ X, const float *__restrict__ Y,
+                                                    const float *__restrict__ Z, const uint8_t *__restrict__ PT,
+                                                    const float *__restrict__ SX, const float *__restrict__ SY,
+                                                    const float *__restrict__ SZ, const SurfDev *__restrict__ surf,
+                                                    double *__restrict__ partials) {
+  // partials are stored quantity-major, partials[a * nmblk + workgroup], so the finalize reads them coalesced
+  const int Lb = blockIdx.x;
+  if (Lb >= nmblk) return;
+  const int4 md = mblk[Lb];  // (lattice, colour, first position, last position + 1)
+  const int r = md.x, colour = md.y;
+  const LatDev &L = lats[r];
+  const float elecE = ctrl[r].elecE;
+  double q[kNQ];
+#pragma unroll
+  for (int a = 0; a < kNQ; a++) q[a] = 0.0;
+  for (int it = 0; it < kMeasPerThread; it++) {
+    const int t = md.z + it * kBlock + (int)threadIdx.x;
+    Site s;
+    uint32_t self;
+    if (t >= md.w || !gather<MASK, SURF>(L, colour, t, X, Y, Z, PT, SX, SY, SZ, s, self)) continue;
+    const V3 n = s.n;
+    q[0] += (double)site_energy<POT, SURF, FIELD>(L, s, n, surf, elecE);
+    const double x = n.x, y = n.y, z = n.z;
+    q[1] += x * x;
+    q[2] += x * y;
+    q[3] += x * z;
+    q[4] += y * y;
+    q[5] += y * z;
+    q[21] += z * z;
+    q[6] += 1.0;
+    if (L.flags & (F_RSPH | F_RCYL)) {
+      const float rx = s.i - L.cx, ry = s.j - L.cy, rz = s.k - L.cz;
+      if (L.flags & F_RSPH) {
+        const float rr = sqrtf(rx * rx + ry * ry + rz * rz);
+        if (rr >= 0.5f) {
+          const double c = (double)(n.x * rx + n.y * ry + n.z * rz) / rr;
+          q[7] += 1.5 * c * c - 0.5;
+          q[8] += c;
+          q[9] += 1.0;
+        }
+      }
+      if (L.flags & F_RCYL) {
+        const float rr = sqrtf(rx * rx + ry * ry);
+        if (rr >= 0.5f) {
+          const double c = (double)(n.x * rx + n.y * ry) / rr;
+          q[10] += 1.5 * c * c - 0.5;
+          q[11] += c;
+          q[12] += 1.0;
+        }
+      }
+    }
+    if constexpr (PEAR) {
+      q[13] += x;
+      q[14] += y;
+      q[15] += z;
+      // splay vector over bonds (Splay_magnitude, parameter_order.cpp:147-176)
+#pragma unroll
+      for (int d = 0; d < 6; d++) {
+        if (!(s.ex & (1u << d))) continue;
+        const V3 b = s.nb[d];
+        const float sg = (d & 1) ? -1.0f : 1.0f;
+        const int ax = d >> 1;
+        const float ra = sg * (ax == 0 ? n.x : (ax == 1 ? n.y : n.z));
+        const float rb = sg * (ax == 0 ? b.x : (ax == 1 ? b.y : b.z));
+        const float ab = dot3(n, b);
+        q[16] += 0.5 * (b.x * rb - n.x * ra + n.x * ab * rb - b.x * ab * ra);
+        q[17] += 0.5 * (b.y * rb - n.y * ra + n.y * ab * rb - b.y * ab * ra);
+        q[18] += 0.5 * (b.z * rb - n.z * ra + n.z * ab * rb - b.z * ab * ra);
+        q[19] += 1.0;
+      }
+      // divergence: central / one-sided differences per axis (Splay_divergence, parameter_order.cpp:183-215)
+      double div = 0.0;
+#pragma unroll
+      for (int ax = 0; ax < 3; ax++) {
+        double dd = 0.0;
+        int nbc = 0;
+        const float own = ax == 0 ? n.x : (ax == 1 ? n.y : n.z);
+        if (s.ex & (1u << (2 * ax))) {
+          const V3 b = s.nb[2 * ax];
+          dd += (double)((ax == 0 ? b.x : (ax == 1 ? b.y : b.z)) - own);
+          nbc++;
+        }
+        if (s.ex & (1u << (2 * ax + 1))) {
+          const V3 b = s.nb[2 * ax + 1];
+          dd -= (double)((ax == 0 ? b.x : (ax == 1 ? b.y : b.z)) - own);
+          nbc++;
+        }
+        if (nbc) div += dd / nbc;
+      }
+      q[20] += div;
+    } else {
+      q[13] += x;
+      q[14] += y;
+      q[15] += z;
+    }
+  }
+  __shared__ double red[kWaves][kNQ];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int a = 0; a < kNQ; a++) {
+    const double v = wave_sum(q[a]);
+    if (lane == 0) red[w][a] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < kNQ) {
+    double v = 0.0;
+#pragma unroll
+    for (int ww = 0; ww < kWaves; ww++) v += red[ww][threadIdx.x];
+    partials[(size_t)threadIdx.x * nmblk + Lb] = v;
+  }
+}
+
+// Largest eigenvalue and eigenvector of a symmetric 3x3 matrix in FP64: trigonometric solution of the
+// characteristic cubic (O. K. Smith, "Eigenvalues of a symmetric 3x3 matrix", CACM 4(4):168, 1961); the
+// eigenvector is the largest cross product of two rows of A - lambda I.
+__device__ double eig_sym3_max(const double a[6], double v[3]) {
+  // a = (xx, xy, xz, yy, yz, zz)
+  const double a00 = a[0], a01 = a[1], a02 = a[2], a11 = a[3], a12 = a[4], a22 = a[5];
+  const double p1 = a01 * a01 + a02 * a02 + a12 * a12;
+  double lam;
+  if (p1 == 0.0) {
+    lam = a00;
+    v[0] = 1; v[1] = 0; v[2] = 0;
+    if (a11 > lam) { lam = a11; v[0] = 0; v[1] = 1; v[2] = 0; }
+    if (a22 > lam) { lam = a22; v[0] = 0; v[1] = 0; v[2] = 1; }
+    return lam;
+  }
+  const double q = (a00 + a11 + a22) / 3.0;
+  const double p2 = (a00 - q) * (a00 - q) + (a11 - q) * (a11 - q) + (a22 - q) * (a22 - q) + 2.0 * p1;
+  const double p = sqrt(p2 / 6.0);
+  const double b00 = (a00 - q) / p, b11 = (a11 - q) / p, b22 = (a22 - q) / p;
+  const double b01 = a01 / p, b02 = a02 / p, b12 = a12 / p;
+  double rr = 0.5 * (b00 * (b11 * b22 - b12 * b12) - b01 * (b01 * b22 - b12 * b02) + b02 * (b01 * b12 - b11 * b02));
+  rr = fmin(1.0, fmax(-1.0, rr));
+  const double phi = acos(rr) / 3.0;
+  lam = q + 2.0 * p * cos(phi);
+  const double r0[3] = {a00 - lam, a01, a02}, r1[3] = {a01, a11 - lam, a12}, r2[3] = {a02, a12, a22 - lam};
+  double c[3][3];
+  c[0][0] = r0[1] * r1[2] - r0[2] * r1[1]; c[0][1] = r0[2] * r1[0] - r0[0] * r1[2]; c[0][2] = r0[0] * r1[1] - r0[1] * r1[0];
+  c[1][0] = r0[1] * r2[2] - r0[2] * r2[1]; c[1][1] = r0[2] * r2[0] - r0[0] * r2[2]; c[1][2] = r0[0] * r2[1] - r0[1] * r2[0];
+  c[2][0] = r1[1] * r2[2] - r1[2] * r2[1]; c[2][1] = r1[2] * r2[0] - r1[0] * r2[2]; c[2][2] = r1[0] * r2[1] - r1[1] * r2[0];
+  int best = 0;
+  double bn = 0.0;
+  for (int x = 0; x < 3; x++) {
+    const double nn = c[x][0] * c[x][0] + c[x][1] * c[x][1] + c[x][2] * c[x][2];
+    if (nn > bn) { bn = nn; best = x; }
+  }
+  if (bn > 0.0) {
+    const double inv = 1.0 / sqrt(bn);
+    v[0] = c[best][0] * inv; v[1] = c[best][1] * inv; v[2] = c[best][2] * inv;
+  } else {
+    v[0] = 0; v[1] = 0; v[2] = 1;
+  }
+  return lam;
+}
+
+// Sums the partials of each lattice in a fixed order (one workgroup per lattice), computes the observables and,
+// when accumulate != 0, adds the sample to the moments.
+__global__ __launch_bounds__(kBlock) void k_finalize(const LatDev *__restrict__ lats,
+                                                     const double *__restrict__ partials, int nmblk,
+                                                     MomDev *__restrict__ mom, ObsDev *__restrict__ obs,
+                                                     int accumulate) {
+  const int r = blockIdx.x;
+  const LatDev &L = lats[r];
+  __shared__ double red[kWaves][kNQ];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  double v[kNQ];
+#pragma unroll
+  for (int a = 0; a < kNQ; a++) v[a] = 0.0;
+  for (int p = L.mp0 + (int)threadIdx.x; p < L.mp1; p += kBlock) {
+#pragma unroll
+    for (int a = 0; a < kNQ; a++) v[a] += partials[(size_t)a * nmblk + p];
+  }
+#pragma unroll
+  for (int a = 0; a < kNQ; a++) {
+    const double x = wave_sum(v[a]);
+    if (lane == 0) red[w][a] = x;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  double tot[kNQ];
+  for (int a = 0; a < kNQ; a++) {
+    double x = 0.0;
+    for (int ww = 0; ww < kWaves; ww++) x += red[ww][a];
+    tot[a] = x;
+  }
+  const double N = tot[6];
+  ObsDev o;
+  o.nvalid = N;
+  const double inv = N > 0.0 ? 1.0 / N : 0.0;
+  o.E = tot[0] * inv;
+  // Q = <(3 n n - I) / 2>
+  o.Q[0] = (3.0 * tot[1] - N) * 0.5 * inv;
+  o.Q[1] = 3.0 * tot[2] * 0.5 * inv;
+  o.Q[2] = 3.0 * tot[3] * 0.5 * inv;
+  o.Q[3] = (3.0 * tot[4] - N) * 0.5 * inv;
+  o.Q[4] = 3.0 * tot[5] * 0.5 * inv;
+  o.Q[5] = (3.0 * tot[21] - N) * 0.5 * inv;
+  // parameter_order.cpp:70 sets Q_zz = -Q_xx - Q_yy (unit directors); the eigen-solve uses the same convention
+  double qm[6] = {o.Q[0], o.Q[1], o.Q[2], o.Q[3], o.Q[4], -o.Q[0] - o.Q[3]};
+  o.S = N > 0.0 ? eig_sym3_max(qm, o.dir) : 0.0;
+  if (N <= 0.0) { o.dir[0] = 0; o.dir[1] = 0; o.dir[2] = 1; }
+  o.srs = tot[9] > 0.0 ? tot[7] / tot[9] : 0.0;
+  o.prs = tot[9] > 0.0 ? tot[8] / tot[9] : 0.0;
+  o.src = tot[12] > 0.0 ? tot[10] / tot[12] : 0.0;
+  o.prc = tot[12] > 0.0 ? tot[11] / tot[12] : 0.0;
+  o.P = fabs(o.dir[0] * tot[13] + o.dir[1] * tot[14] + o.dir[2] * tot[15]) * inv;
+  o.dth = tot[19] > 0.0 ? 1.5 * sqrt(tot[16] * tot[16] + tot[17] * tot[17] + tot[18] * tot[18]) / tot[19] : 0.0;
+  o.divn = tot[20] * inv;
+  obs[r] = o;
+  if (accumulate) {
+    MomDev m = mom[r];
+    m.n += 1.0;
+    m.S += o.S;
+    m.S2 += o.S * o.S;
+    m.E += o.E;
+    m.E2 += o.E * o.E;
+    m.srs += o.srs;
+    m.prs += o.prs;
+    m.src += o.src;
+    m.prc += o.prc;
+    m.P += o.P;
+    m.dth += o.dth;
+    m.divn += o.divn;
+    mom[r] = m;
+  }
+}
+
+__global__ void k_zero_moments(MomDev *mom, int n) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < n) mom[r] = MomDev{0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+}
+
+// Local order parameter of every site of one lattice (lattice_order_parameter, parameter_order.cpp:75-127):
+// Q over the site and its non-empty neighbours, largest eigenvalue in FP64; NaN on empty sites. Output in the
+// AoS site order idx = i + nx (j + ny k).
+template <bool MASK>
+__global__ __launch_bounds__(kBlock) void k_local_order(const LatDev *__restrict__ lats, int r,
+                                                        const float *__restrict__ X, const float *__restrict__ Y,
+                                                        const float *__restrict__ Z, const uint8_t *__restrict__ PT,
+                                                        float *__restrict__ out) {
+  const LatDev &L = lats[r];
+  const int g = blockIdx.x * kBlock + threadIdx.x;
+  if (g >= 2 * L.nc) return;
+  const int colour = g >= L.nc ? 1 : 0;
+  const int t = g - colour * L.nc;
+  Site s;
+  uint32_t self;
+  const uint32_t row = fdiv((uint32_t)t, L.mxh, L.sxh);
+  const int m = t - (int)row * L.nxh;
+  const int k = (int)fdiv(row, L.my, L.sy);
+  const int j = (int)row - k * L.ny;
+  const int i = 2 * m + ((j + k + colour) & 1);
+  if (i >= L.nx) return;
+  const long long lin = (long long)i + (long long)L.nx * (j + (long long)L.ny * k);
+  if (!gather<MASK, false>(L, colour, t, X, Y, Z, PT, nullptr, nullptr, nullptr, s, self)) {
+    out[lin] = __int_as_float(0x7fc00000);
+    return;
+  }
+  double a[6] = {0, 0, 0, 0, 0, 0};
+  int pts = 0;
+  auto add = [&](V3 v) {
+    a[0] += 3.0 * (double)(v.x * v.x) - 1.0;
+    a[1] += 3.0 * (double)(v.x * v.y);
+    a[2] += 3.0 * (double)(v.x * v.z);
+    a[3] += 3.0 * (double)(v.y * v.y) - 1.0;
+    a[4] += 3.0 * (double)(v.y * v.z);
+    a[5] += 3.0 * (double)(v.z * v.z) - 1.0;
+    pts++;
+  };
+  add(s.n);
+  for (int d = 0; d < 6; d++)
+    if (s.ex & (1u << d)) add(s.nb[d]);
+  for (int x = 0; x < 6; x++) a[x] /= 2.0 * pts;
+  double v[3];
+  out[lin] = (float)eig_sym3_max(a, v);
+}
+
+// Energies of given sites with their current director and with a trial director, by the sweep's energy function.
+template <int POT, bool MASK, bool SURF, bool FIELD>
+__global__ void k_site_energies(const LatDev *__restrict__ lats, const CtrlDev *__restrict__ ctrl, int r,
+                                const int *__restrict__ sites, const float *__restrict__ trial, int n,
+                                const float *__restrict__ X, const float *__restrict__ Y,
+                                const float *__restrict__ Z, const uint8_t *__restrict__ PT,
+                                const float *__restrict__ SX, const float *__restrict__ SY,
+                                const float *__restrict__ SZ, const SurfDev *__restrict__ surf,
+                                float *__restrict__ e_cur, float *__restrict__ e_trial) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n) return;
+  const LatDev &L = lats[r];
+  const int lin = sites[g];
+  const int i = lin % L.nx, j = (lin / L.nx) % L.ny, k = lin / (L.nx * L.ny);
+  const int colour = (i + j + k) & 1;
+  const int t = (i >> 1) + L.nxh * (j + L.ny * k);
+  Site s;
+  uint32_t self;
+  if (!gather<MASK, SURF>(L, colour, t, X, Y, Z, PT, SX, SY, SZ, s, self)) {
+    e_cur[g] = 0.0f;
+    e_trial[g] = 0.0f;
+    return;
+  }
+  const float elecE = ctrl[r].elecE;
+  e_cur[g] = site_energy<POT, SURF, FIELD>(L, s, s.n, surf, elecE);
+  const V3 tv{trial[3 * g], trial[3 * g + 1], trial[3 * g + 2]};
+  e_trial[g] = site_energy<POT, SURF, FIELD>(L, s, tv, surf, elecE);
+}
+
+__global__ void k_philox(const uint32_t *__restrict__ ctr, const uint32_t *__restrict__ key, int n,
+                         uint32_t *__restrict__ out) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n) return;
+  const mcl::Philox4 R =
+      mcl::philox4x32_10(ctr[4 * g], ctr[4 * g + 1], ctr[4 * g + 2], ctr[4 * g + 3], key[2 * g], key[2 * g + 1]);
+  for (int x = 0; x < 4; x++) out[4 * g + x] = R.v[x];
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// Host side.
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string &msg) {
+  g_last_error = msg;
+  return code;
+}
+
+#define MCL_HIP(call)                                                                                 \
+  do {                                                                                                \
+    hipError_t e_ = (call);                                                                           \
+    if (e_ != hipSuccess)                                                                             \
+      return fail(e_ == hipErrorOutOfMemory ? MCL_ERR_NOMEM : MCL_ERR_DEVICE,                         \
+                  std::string(#call) + ": " + hipGetErrorString(e_));                                 \
+  } while (0)
+
+struct LatHost {
+  int nx, ny, nz, nxh, nc;
+  long long off;
+  int nvalid;
+  bool mask, surf;
+  int radial;
+};
+
+struct Ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int nlat = 0;
+  int pot = 0;
+  bool mask = false, surf = false, field = false, flip = false;
+  std::vector<LatHost> lh;
+  std::vector<LatDev> ld;
+  long long pool = 0;  // sites in the pooled colour blocks
+  float *X = nullptr, *Y = nullptr, *Z = nullptr;
+  uint8_t *PT = nullptr;
+  float *SX = nullptr, *SY = nullptr, *SZ = nullptr;
+  SurfDev *surf_d = nullptr;
+  LatDev *lat_d = nullptr;
+  CtrlDev *ctrl_d = nullptr;
+  int2 *blk_d = nullptr;
+  unsigned int *bcount = nullptr;  // accepted rotations per sweep workgroup, [colour][workgroup]
+  int nblk = 0, chunk = 0;
+  int4 *mblk_d = nullptr;
+  int nmblk = 0;
+  double *partials = nullptr;
+  MomDev *mom_d = nullptr;
+  ObsDev *obs_d = nullptr;
+  float *scratch_f = nullptr;  // Nt floats of the largest lattice (local order output)
+  long long max_nt = 0;
+  // kernel timing
+  int timing_stride = 0;
+  std::vector<hipEvent_t> ev;
+  size_t ev_used = 0;
+  std::vector<std::pair<size_t, size_t>> ev_pairs;
+};
+
+using SweepFn = void (*)(dim3, dim3, hipStream_t, const Ctx &, int);
+using MeasFn = void (*)(dim3, dim3, hipStream_t, const Ctx &);
+
+template <int POT, bool MASK, bool SURF, bool FIELD, bool FLIP>
+void launch_sweep(dim3 g, dim3 b, hipStream_t st, const Ctx &c, int colour) {
+  hipLaunchKernelGGL((k_sweep<POT, MASK, SURF, FIELD, FLIP>), g, b, 0, st, c.lat_d, c.ctrl_d, c.bcount, c.blk_d,
+                     c.nblk, c.chunk, colour, c.X, c.Y, c.Z, c.PT, c.SX, c.SY, c.SZ, c.surf_d);
+}
+template <int POT, bool MASK, bool SURF, bool FIELD>
+void launch_measure(dim3 g, dim3 b, hipStream_t st, const Ctx &c) {
+  hipLaunchKernelGGL((k_measure<POT, MASK, SURF, FIELD, POT == MCL_POT_PEAR>), g, b, 0, st, c.lat_d, c.ctrl_d,
+                     c.mblk_d, c.nmblk, c.X, c.Y, c.Z, c.PT, c.SX, c.SY, c.SZ, c.surf_d, c.partials);
+}
+
+template <int POT, bool MASK, bool SURF, bool FIELD>
+SweepFn pick_sweep_flip(bool flip) {
+  return flip ? launch_sweep<POT, MASK, SURF, FIELD, true> : launch_sweep<POT, MASK, SURF, FIELD, false>;
+}
+template <int POT, bool MASK, bool SURF>
+SweepFn pick_sweep_field(bool field, bool flip) {
+  return field ? pick_sweep_flip<POT, MASK, SURF, true>(flip) : pick_sweep_flip<POT, MASK, SURF, false>(flip);
+}
+template <int POT>
+SweepFn pick_sweep_geom(bool mask, bool surf, bool field, bool flip) {
+  if (surf) return pick_sweep_field<POT, true, true>(field, flip);
+  if (mask) return pick_sweep_field<POT, true, false>(field, flip);
+  return pick_sweep_field<POT, false, false>(field, flip);
+}
+SweepFn pick_sweep(const Ctx &c) {
+  switch (c.pot) {
+    case MCL_POT_GHRL: return pick_sweep_geom<MCL_POT_GHRL>(c.mask, c.surf, c.field, c.flip);
+    case MCL_POT_PEAR: return pick_sweep_geom<MCL_POT_PEAR>(c.mask, c.surf, c.field, c.flip);
+    default: return pick_sweep_geom<MCL_POT_LL>(c.mask, c.surf, c.field, c.flip);
+  }
+}
+
+template <int POT, bool MASK, bool SURF>
+MeasFn pick_meas_field(bool field) {
+  return field ? launch_measure<POT, MASK, SURF, true> : launch_measure<POT, MASK, SURF, false>;
+}
+template <int POT>
+MeasFn pick_meas_geom(bool mask, bool surf, bool field) {
+  if (surf) return pick_meas_field<POT, true, true>(field);
+  if (mask) return pick_meas_field<POT, true, false>(field);
+  return pick_meas_field<POT, false, false>(field);
+}
+MeasFn pick_meas(const Ctx &c) {
+  switch (c.pot) {
+    case MCL_POT_GHRL: return pick_meas_geom<MCL_POT_GHRL>(c.mask, c.surf, c.field);
+    case MCL_POT_PEAR: return pick_meas_geom<MCL_POT_PEAR>(c.mask, c.surf, c.field);
+    default: return pick_meas_geom<MCL_POT_LL>(c.mask, c.surf, c.field);
+  }
+}
+
+Ctx *as_ctx(void *p) { return static_cast<Ctx *>(p); }
+
+int check_lat(Ctx *c, int lat) {
+  if (!c) return fail(MCL_ERR_ARG, "null context");
+  if (lat < 0 || lat >= c->nlat) return fail(MCL_ERR_ARG, "lattice index out of range");
+  return MCL_OK;
+}
+
+// AoS host directors <-> pooled colour-split SoA of lattice r.
+void aos_to_soa(const LatHost &h, const float *ni, std::vector<float> &x, std::vector<float> &y,
+                std::vector<float> &z) {
+  x.assign(2 * (size_t)h.nc, 0.0f);
+  y.assign(2 * (size_t)h.nc, 0.0f);
+  z.assign(2 * (size_t)h.nc, 1.0f);
+  for (int k = 0; k < h.nz; k++)
+    for (int j = 0; j < h.ny; j++)
+      for (int i = 0; i < h.nx; i++) {
+        const size_t lin = (size_t)i + (size_t)h.nx * (j + (size_t)h.ny * k);
+        const int colour = (i + j + k) & 1;
+        const size_t t = (size_t)colour * h.nc + (i >> 1) + (size_t)h.nxh * (j + (size_t)h.ny * k);
+        x[t] = ni[3 * lin];
+        y[t] = ni[3 * lin + 1];
+        z[t] = ni[3 * lin + 2];
+      }
+}
+
+int upload_directors(Ctx *c, int r, const float *ni) {
+  const LatHost &h = c->lh[r];
+  std::vector<float> x, y, z;
+  aos_to_soa(h, ni, x, y, z);
+  const size_t bytes = sizeof(float) * 2 * (size_t)h.nc;
+  MCL_HIP(hipMemcpyAsync(c->X + h.off, x.data(), bytes, hipMemcpyHostToDevice, c->stream));
+  MCL_HIP(hipMemcpyAsync(c->Y + h.off, y.data(), bytes, hipMemcpyHostToDevice, c->stream));
+  MCL_HIP(hipMemcpyAsync(c->Z + h.off, z.data(), bytes, hipMemcpyHostToDevice, c->stream));
+  MCL_HIP(hipStreamSynchronize(c->stream));
+  return MCL_OK;
+}
+
+void destroy_ctx(Ctx *c) {
+  if (!c) return;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (auto e : c->ev) (void)hipEventDestroy(e);
+  void *bufs[] = {c->X, c->Y, c->Z, c->PT, c->SX, c->SY, c->SZ, c->surf_d, c->lat_d, c->ctrl_d,
+                  c->blk_d, c->bcount, c->mblk_d, c->partials, c->mom_d, c->obs_d, c->scratch_f};
+  for (void *b : bufs)
+    if (b) (void)hipFree(b);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+template <typename T>
+int dev_alloc(T **p, size_t n) {
+  void *q = nullptr;
+  hipError_t e = hipMalloc(&q, sizeof(T) * (n ? n : 1));
+  if (e != hipSuccess) return fail(MCL_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+  *p = static_cast<T *>(q);
+  return MCL_OK;
+}
+
+int create_ctx(const mcl_lattice_desc *lat, int nlat, Ctx *c) {
+  if (!lat || nlat < 1) return fail(MCL_ERR_ARG, "no lattices");
+  MCL_HIP(hipGetDevice(&c->device));
+  MCL_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  c->nlat = nlat;
+  c->pot = lat[0].model.potential;
+  c->flip = lat[0].model.flip_move != 0;
+  if (c->pot < MCL_POT_LL || c->pot > MCL_POT_PEAR) return fail(MCL_ERR_ARG, "unknown potential");
+  std::vector<SurfDev> surf_h;
+  c->lh.resize(nlat);
+  c->ld.resize(nlat);
+  long long off = 0;
+  for (int r = 0; r < nlat; r++) {
+    const mcl_lattice_desc &d = lat[r];
+    if (d.model.potential != c->pot || (d.model.flip_move != 0) != c->flip)
+      return fail(MCL_ERR_ARG, "potential and flip_move must be the same for every lattice of a context");
+    if (d.nx < 1 || d.ny < 1 || d.nz < 1) return fail(MCL_ERR_ARG, "grid sizes must be >= 1");
+    if ((long long)d.nx * d.ny * d.nz >= (1ll << 31)) return fail(MCL_ERR_ARG, "lattice too large (>= 2^31 sites)");
+    const int n3[3] = {d.nx, d.ny, d.nz};
+    for (int a = 0; a < 3; a++)
+      if (d.periodic[a] && (n3[a] % 2))
+        return fail(MCL_ERR_ARG, "a periodic axis needs an even number of sites (checkerboard update)");
+    if (!d.ni) return fail(MCL_ERR_ARG, "null director array");
+    if (d.nsurf < 0 || d.nsurf > 253) return fail(MCL_ERR_ARG, "nsurf must be in [0, 253]");
+    if (d.nsurf > 0 && (!d.surf || !d.ns)) return fail(MCL_ERR_ARG, "surfaces need the surface table and normals");
+    LatHost &h = c->lh[r];
+    h.nx = d.nx;
+    h.ny = d.ny;
+    h.nz = d.nz;
+    h.nxh = (d.nx + 1) / 2;
+    h.nc = h.nxh * d.ny * d.nz;
+    h.off = off;
+    off += 2ll * h.nc;
+    const long long Nt = (long long)d.nx * d.ny * d.nz;
+    c->max_nt = std::max(c->max_nt, Nt);
+    h.nvalid = 0;
+    h.mask = false;
+    h.surf = false;
+    if (d.pt) {
+      for (long long s = 0; s < Nt; s++) {
+        const int p = d.pt[s];
+        if (p < 0 || p > 255) return fail(MCL_ERR_ARG, "point type out of range [0, 255]");
+        if (p >= 2 && p - 2 >= d.nsurf) return fail(MCL_ERR_ARG, "point type names a surface without anchoring");
+        h.nvalid += p ? 1 : 0;
+        if (p != 1) h.mask = true;
+        if (p >= 2) h.surf = true;
+      }
+    } else {
+      h.nvalid = (int)Nt;
+    }
+    h.radial = d.radial;
+    LatDev &L = c->ld[r];
+    std::memset(&L, 0, sizeof(L));
+    L.nx = d.nx;
+    L.ny = d.ny;
+    L.nz = d.nz;
+    L.nxh = h.nxh;
+    L.nc = h.nc;
+    const FastDiv fx = make_fastdiv((uint32_t)h.nxh), fy = make_fastdiv((uint32_t)d.ny);
+    L.mxh = fx.m;
+    L.sxh = fx.sh;
+    L.my = fy.m;
+    L.sy = fy.sh;
+    L.flags = (d.periodic[0] ? F_PX : 0) | (d.periodic[1] ? F_PY : 0) | (d.periodic[2] ? F_PZ : 0) |
+              (h.mask ? F_MASK : 0) | (h.surf ? F_SURF : 0) | ((d.radial & 1) ? F_RSPH : 0) |
+              ((d.radial & 2) ? F_RCYL : 0);
+    L.nvalid = h.nvalid;
+    L.off = (uint32_t)h.off;
+    L.key0 = (uint32_t)d.seed;
+    L.key1 = (uint32_t)(d.seed >> 32) ^ (0x9E3779B9u * (uint32_t)(r + 1));
+    L.nblk = (h.nc + kBlock - 1) / kBlock;
+    L.surf0 = (int)surf_h.size();
+    const mcl_model &M = d.model;
+    L.A = M.A;
+    L.B1 = M.B1;
+    L.C = M.C;
+    L.gl = M.ghrl_lambda;
+    L.gm = M.ghrl_mu;
+    L.gn = M.ghrl_nu;
+    L.gr = M.ghrl_rho;
+    L.gs = M.ghrl_sigma;
+    L.ex = M.elec_dir[0];
+    L.ey = M.elec_dir[1];
+    L.ez = M.elec_dir[2];
+    L.elecA = M.elecA;
+    L.elecP = M.elecP;
+    L.cx = (float)(d.nx / 2);
+    L.cy = (float)(d.ny / 2);
+    L.cz = (float)(d.nz / 2);
+    if (M.elecA != 0.0f || M.elecP != 0.0f) c->field = true;
+    c->mask |= h.mask;
+    c->surf |= h.surf;
+    for (int s = 0; s < d.nsurf; s++) {
+      const mcl_surface &S = d.surf[s];
+      SurfDev sd{0, S.W, S.axis[0], S.axis[1], S.axis[2]};
+      switch (S.type) {
+        case MCL_ANCH_HOMEOTROPIC:
+        case MCL_ANCH_RP:
+        case MCL_ANCH_RP_TANGENT: sd.kind = 0; break;
+        case MCL_ANCH_FG: sd.kind = 1; break;
+        case MCL_ANCH_HOMEOTROPIC_GHRL: sd.kind = 2; break;
+        case MCL_ANCH_RP_GHRL: sd.kind = 3; break;
+        case MCL_ANCH_FG_GHRL: sd.kind = 4; break;
+        default: return fail(MCL_ERR_ARG, "unknown anchoring type");
+      }
+      surf_h.push_back(sd);
+    }
+  }
+  c->pool = off;
+  // sweep work list: per lattice ceil(nc / kBlock) workgroups of kBlock consecutive colour positions
+  if (off >= (1ll << 31)) return fail(MCL_ERR_ARG, "too many sites in one context (>= 2^31)");
+  std::vector<int2> blk;
+  for (int r = 0; r < nlat; r++) {
+    c->ld[r].blk0 = (int)blk.size();
+    for (int b = 0; b < c->ld[r].nblk; b++) blk.push_back(make_int2(r, b * kBlock));
+  }
+  c->nblk = (int)blk.size();
+  c->chunk = (c->nblk + 7) / 8;
+  // measurement work list: per lattice and colour, workgroups of kBlock * kMeasPerThread positions
+  std::vector<int4> mblk;
+  const int span = kBlock * kMeasPerThread;
+  for (int r = 0; r < nlat; r++) {
+    c->ld[r].mp0 = (int)mblk.size();
+    for (int col = 0; col < 2; col++)
+      for (int t0 = 0; t0 < c->ld[r].nc; t0 += span)
+        mblk.push_back(make_int4(r, col, t0, std::min(t0 + span, c->ld[r].nc)));
+    c->ld[r].mp1 = (int)mblk.size();
+  }
+  c->nmblk = (int)mblk.size();
+
+  int st;
+  if ((st = dev_alloc(&c->X, c->pool)) || (st = dev_alloc(&c->Y, c->pool)) || (st = dev_alloc(&c->Z, c->pool)))
+    return st;
+  if (c->mask && (st = dev_alloc(&c->PT, c->pool))) return st;
+  if (c->surf && ((st = dev_alloc(&c->SX, c->pool)) || (st = dev_alloc(&c->SY, c->pool)) ||
+                  (st = dev_alloc(&c->SZ, c->pool))))
+    return st;
+  if ((st = dev_alloc(&c->surf_d, surf_h.size())) || (st = dev_alloc(&c->lat_d, nlat)) ||
+      (st = dev_alloc(&c->ctrl_d, nlat)) || (st = dev_alloc(&c->blk_d, blk.size())) ||
+      (st = dev_alloc(&c->bcount, 2 * blk.size())) ||
+      (st = dev_alloc(&c->mblk_d, mblk.size())) || (st = dev_alloc(&c->partials, mblk.size() * kNQ)) ||
+      (st = dev_alloc(&c->mom_d, nlat)) || (st = dev_alloc(&c->obs_d, nlat)) ||
+      (st = dev_alloc(&c->scratch_f, (size_t)c->max_nt)))
+    return st;
+  if (!surf_h.empty())
+    MCL_HIP(hipMemcpyAsync(c->surf_d, surf_h.data(), sizeof(SurfDev) * surf_h.size(), hipMemcpyHostToDevice,
+                           c->stream));
+  MCL_HIP(hipMemcpyAsync(c->lat_d, c->ld.data(), sizeof(LatDev) * nlat, hipMemcpyHostToDevice, c->stream));
+  MCL_HIP(hipMemcpyAsync(c->blk_d, blk.data(), sizeof(int2) * blk.size(), hipMemcpyHostToDevice, c->stream));
+  MCL_HIP(hipMemcpyAsync(c->mblk_d, mblk.data(), sizeof(int4) * mblk.size(), hipMemcpyHostToDevice, c->stream));
+  std::vector<CtrlDev> ctrl(nlat);
+  for (int r = 0; r < nlat; r++) {
+    ctrl[r] = CtrlDev{lat[r].T, lat[r].model.elecE, lat[r].ang_var, 0u, 0ull};
+  }
+  MCL_HIP(hipMemcpyAsync(c->ctrl_d, ctrl.data(), sizeof(CtrlDev) * nlat, hipMemcpyHostToDevice, c->stream));
+  std::vector<MomDev> mom(nlat, MomDev{0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0});
+  MCL_HIP(hipMemcpyAsync(c->mom_d, mom.data(), sizeof(MomDev) * nlat, hipMemcpyHostToDevice, c->stream));
+  // per-lattice point types and anchoring vectors in the colour-split layout
+  for (int r = 0; r < nlat; r++) {
+    const mcl_lattice_desc &d = lat[r];
+    const LatHost &h = c->lh[r];
+    if (c->mask) {
+      std::vector<uint8_t> p(2 * (size_t)h.nc, 0);
+      for (int k = 0; k < h.nz; k++)
+        for (int j = 0; j < h.ny; j++)
+          for (int i = 0; i < h.nx; i++) {
+            const size_t lin = (size_t)i + (size_t)h.nx * (j + (size_t)h.ny * k);
+            const size_t t = (size_t)((i + j + k) & 1) * h.nc + (i >> 1) + (size_t)h.nxh * (j + (size_t)h.ny * k);
+            p[t] = d.pt ? (uint8_t)d.pt[lin] : (uint8_t)1;
+          }
+      MCL_HIP(hipMemcpy(c->PT + h.off, p.data(), p.size(), hipMemcpyHostToDevice));
+    }
+    if (c->surf) {
+      // per-site vector of the anchoring kernels: the normal, or the easy axis for rp and rp_tangent
+      // (evolve.cu:348-391)
+      const long long Nt = (long long)h.nx * h.ny * h.nz;
+      std::vector<float> sv(3 * (size_t)Nt, 0.0f);
+      if (h.surf) {
+        for (long long s = 0; s < Nt; s++) {
+          const int p = d.pt[s];
+          if (p < 2) continue;
+          const mcl_surface &S = d.surf[p - 2];
+          const float *nrm = &d.ns[3 * s];
+          float *o = &sv[3 * s];
+          if (S.type == MCL_ANCH_RP) {
+            o[0] = S.axis[0];
+            o[1] = S.axis[1];
+            o[2] = S.axis[2];
+          } else if (S.type == MCL_ANCH_RP_TANGENT) {
+            const float ds = S.axis[0] * nrm[0] + S.axis[1] * nrm[1] + S.axis[2] * nrm[2];
+            float e[3] = {S.axis[0] - ds * nrm[0], S.axis[1] - ds * nrm[1], S.axis[2] - ds * nrm[2]};
+            const float mod = sqrtf(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]);
+            if (mod < 1e-3f) {  // RP_Tangent_Anchoring::singular_tol: no anchoring at this site
+              o[0] = o[1] = o[2] = 0.0f;
+            } else {
+              o[0] = e[0] / mod;
+              o[1] = e[1] / mod;
+              o[2] = e[2] / mod;
+            }
+          } else {
+            o[0] = nrm[0];
+            o[1] = nrm[1];
+            o[2] = nrm[2];
+          }
+        }
+      }
+      std::vector<float> x, y, z;
+      aos_to_soa(h, sv.data(), x, y, z);
+      const size_t bytes = sizeof(float) * 2 * (size_t)h.nc;
+      MCL_HIP(hipMemcpy(c->SX + h.off, x.data(), bytes, hipMemcpyHostToDevice));
+      MCL_HIP(hipMemcpy(c->SY + h.off, y.data(), bytes, hipMemcpyHostToDevice));
+      MCL_HIP(hipMemcpy(c->SZ + h.off, z.data(), bytes, hipMemcpyHostToDevice));
+    }
+    if ((st = upload_directors(c, r, d.ni))) return st;
+  }
+  MCL_HIP(hipStreamSynchronize(c->stream));
+  return MCL_OK;
+}
+
+hipEvent_t next_event(Ctx *c) {
+  if (c->ev_used == c->ev.size()) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    c->ev.push_back(e);
+  }
+  return c->ev[c->ev_used++];
+}
+
+int run_measure(Ctx *c, int accumulate) {
+  const MeasFn mf = pick_meas(*c);
+  mf(dim3(c->nmblk), dim3(kBlock), c->stream, *c);
+  hipLaunchKernelGGL(k_finalize, dim3(c->nlat), dim3(kBlock), 0, c->stream, c->lat_d, c->partials, c->nmblk,
+                     c->mom_d, c->obs_d, accumulate);
+  MCL_HIP(hipGetLastError());
+  return MCL_OK;
+}
+
+template <int POT, bool MASK, bool SURF, bool FIELD>
+void launch_site_energies(const Ctx *c, int r, const int *sites, const float *trial, int n, float *ec,
+                                 float *et) {
+  hipLaunchKernelGGL((k_site_energies<POT, MASK, SURF, FIELD>), dim3((n + 255) / 256), dim3(256), 0, c->stream,
+                     c->lat_d, c->ctrl_d, r, sites, trial, n, c->X, c->Y, c->Z, c->PT, c->SX, c->SY, c->SZ,
+                     c->surf_d, ec, et);
+}
+template <int POT>
+void dispatch_site_energies(const Ctx *c, int r, const int *sites, const float *trial, int n, float *ec,
+                                   float *et) {
+  if (c->surf) {
+    if (c->field) launch_site_energies<POT, true, true, true>(c, r, sites, trial, n, ec, et);
+    else launch_site_energies<POT, true, true, false>(c, r, sites, trial, n, ec, et);
+  } else if (c->mask) {
+    if (c->field) launch_site_energies<POT, true, false, true>(c, r, sites, trial, n, ec, et);
+    else launch_site_energies<POT, true, false, false>(c, r, sites, trial, n, ec, et);
+  } else {
+    if (c->field) launch_site_energies<POT, false, false, true>(c, r, sites, trial, n, ec, et);
+    else launch_site_energies<POT, false, false, false>(c, r, sites, trial, n, ec, et);
+  }
+}
+
+int set_ctrl_float(Ctx *c, int32_t lat, size_t field_off, float v) {
+  if (!c) return fail(MCL_ERR_ARG, "null context");
+  if (lat >= c->nlat) return fail(MCL_ERR_ARG, "lattice index out of range");
+  const int r0 = lat < 0 ? 0 : lat, r1 = lat < 0 ? c->nlat : lat + 1;
+  for (int r = r0; r < r1; r++) {
+    char *dst = reinterpret_cast<char *>(c->ctrl_d + r) + field_off;
+    MCL_HIP(hipMemcpyAsync(dst, &v, sizeof(float), hipMemcpyHostToDevice, c->stream));
+  }
+  MCL_HIP(hipStreamSynchronize(c->stream));
+  return MCL_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------------------------------------
+// C ABI
+
+extern "C" {
+
+int mcl_abi_version(void) { return MCL_ABI_VERSION; }
+
+const char *mcl_last_error(void) { return g_last_error.c_str(); }
+
+int mcl_create(const mcl_lattice_desc *lat, int32_t nlat, void **ctx_out) {
+  if (!ctx_out) return fail(MCL_ERR_ARG, "null output pointer");
+  *ctx_out = nullptr;
+  Ctx *c = new Ctx();
+  const int st = create_ctx(lat, nlat, c);
+  if (st != MCL_OK) {
+    const std::string msg = g_last_error;
+    destroy_ctx(c);
+    g_last_error = msg;
+    return st;
+  }
+  *ctx_out = c;
+  return MCL_OK;
+}
+
+void mcl_destroy(void *ctx) { destroy_ctx(as_ctx(ctx)); }
+
+int mcl_sweeps(void *ctx, int32_t nsweeps, int32_t measure_every) {
+  Ctx *c = as_ctx(ctx);
+  if (!c) return fail(MCL_ERR_ARG, "null context");
+  if (nsweeps < 0 || measure_every < 0) return fail(MCL_ERR_ARG, "negative sweep count");
+  const SweepFn sf = pick_sweep(*c);
+  const dim3 grid(8 * c->chunk), block(kBlock);
+  for (int s = 0; s < nsweeps; s++) {
+    const bool timed = c->timing_stride > 0 && (s % c->timing_stride) == 0;
+    if (timed) {
+      hipEvent_t e0 = next_event(c), e1 = next_event(c), e2 = next_event(c);
+      if (!e0 || !e1 || !e2) return fail(MCL_ERR_DEVICE, "hipEventCreate failed");
+      MCL_HIP(hipEventRecord(e0, c->stream));
+      sf(grid, block, c->stream, *c, 0);
+      MCL_HIP(hipEventRecord(e1, c->stream));
+      sf(grid, block, c->stream, *c, 1);
+      MCL_HIP(hipEventRecord(e2, c->stream));
+      c->ev_pairs.push_back({c->ev_used - 3, c->ev_used - 2});
+      c->ev_pairs.push_back({c->ev_used - 2, c->ev_used - 1});
+    } else {
+      sf(grid, block, c->stream, *c, 0);
+      sf(grid, block, c->stream, *c, 1);
+    }
+    hipLaunchKernelGGL(k_close_sweep, dim3(c->nlat), dim3(kBlock), 0, c->stream, c->lat_d, c->ctrl_d, c->bcount,
+                       c->nblk);
+    if (measure_every > 0 && (s + 1) % measure_every == 0) {
+      const int st = run_measure(c, 1);
+      if (st) return st;
+    }
+  }
+  MCL_HIP(hipGetLastError());
+  return MCL_OK;
+}
+
+int mcl_set_kernel_timing(void *ctx, int32_t stride, int32_t max_timed_sweeps) {
+  Ctx *c = as_ctx(ctx);
+  if (!c) return fail(MCL_ERR_ARG, "null context");
+  MCL_HIP(hipStreamSynchronize(c->stream));
+  c->timing_stride = stride > 0 ? stride : 0;
+  c->ev_used = 0;
+  c->ev_pairs.clear();
+  // create the events up front so that no hipEventCreate falls inside a timed region
+  while (c->ev.size() < 3 * (size_t)std::max(0, max_timed_sweeps)) {
+    hipEvent_t e;
+    MCL_HIP(hipEventCreate(&e));
+    c->ev.push_back(e);
+  }
+  return MCL_OK;
+}
+
+int mcl_kernel_timing(void *ctx, int64_t *launches, double *total_ms) {
+  Ctx *c = as_ctx(ctx);
+  if (!c || !launches || !total_ms) return fail(MCL_ERR_ARG, "null argument");
+  MCL_HIP(hipStreamSynchronize(c->stream));
+  double tot = 0.0;
+  for (auto &p : c->ev_pairs) {
+    float ms = 0.0f;
+    MCL_HIP(hipEventElapsedTime(&ms, c->ev[p.first], c->ev[p.second]));
+    tot += ms;
+  }
+  *launches = (int64_t)c->ev_pairs.size();
+  *total_ms = tot;
+  return MCL_OK;
+}
+
+int mcl_get_stream(void *ctx, void **stream) {
+  Ctx *c = as_ctx(ctx);
+  if (!c || !stream) return fail(MCL_ERR_ARG, "null argument");
+  *stream = (void *)c->stream;
+  return MCL_OK;
+}
+
+int mcl_set_temperature(void *ctx, int32_t lat, float T) {
+  if (T < 0.0f) return fail(MCL_ERR_ARG, "negative temperature");
+  return set_ctrl_float(as_ctx(ctx), lat, offsetof(CtrlDev, T), T);
+}
+int mcl_set_field(void *ctx, int32_t lat, float elecE) {
+  return set_ctrl_float(as_ctx(ctx), lat, offsetof(CtrlDev, elecE), elecE);
+}
+int mcl_set_ang_var(void *ctx, int32_t lat, float ang_var) {
+  return set_ctrl_float(as_ctx(ctx), lat, offsetof(CtrlDev, ang_var), ang_var);
+}
+
+int mcl_get_ang_var(void *ctx, int32_t lat, float *ang_var) {
+  Ctx *c = as_ctx(ctx);
+  int st = check_lat(c, lat);
+  if (st) return st;
+  if (!ang_var) return fail(MCL_ERR_ARG, "null argument");
+  CtrlDev h;
+  MCL_HIP(hipMemcpyAsync(&h, c->ctrl_d + lat, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+  MCL_HIP(hipStreamSynchronize(c->stream));
+  *ang_var = h.ang_var;
+  return MCL_OK;
+}
+
+int mcl_get_sweep_stats(void *ctx, int32_t lat, int64_t *last_accepted, int64_t *sweeps_done) {
+  Ctx *c = as_ctx(ctx);
+  int st = check_lat(c, lat);
+  if (st) return st;
+  CtrlDev h;
+  MCL_HIP(hipMemcpyAsync(&h, c->ctrl_d + lat, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+  MCL_HIP(hipStreamSynchronize(c->stream));
+  if (last_accepted) *last_accepted = (int64_t)h.last_acc;
+  if (sweeps_done) *sweeps_done = (int64_t)h.sweep;
+  return MCL_OK;
+}
+
+int mcl_reset_moments(void *ctx) {
+  Ctx *c = as_ctx(ctx);
+  if (!c) return fail(MCL_ERR_ARG, "null context");
+  hipLaunchKernelGGL(k_zero_moments, dim3((c->nlat + 255) / 256), dim3(256), 0, c->stream, c->mom_d, c->nlat);
+  MCL_HIP(hipGetLastError());
+  return MCL_OK;
+}
+
+int mcl_get_moments(void *ctx, int32_t lat, mcl_moments *out) {
+  Ctx *c = as_ctx(ctx);
+  int st = check_lat(c, lat);
+  if (st) return st;
+  if (!out) return fail(MCL_ERR_ARG, "null argument");
+  MomDev m;
+  MCL_HIP(hipMemcpyAsync(&m, c->mom_d + lat, sizeof(m), hipMemcpyDeviceToHost, c->stream));
+  MCL_HIP(hipStreamSynchronize(c->stream));
+  out->n = (int64_t)m.n;
+  out->S = m.S;
+  out->S2 = m.S2;
+  out->E = m.E;
+  out->E2 = m.E2;
+  out->Sr_sph = m.srs;
+  out->Pr_sph = m.prs;
+  out->Sr_cyl = m.src;
+  out->Pr_cyl = m.prc;
+  out->P = m.P;
+  out->dtheta = m.dth;
+  out->divn = m.divn;
+  return MCL_OK;
+}
+
+int mcl_measure(void *ctx, int32_t lat, mcl_observables *out) {
+  Ctx *c = as_ctx(ctx);
+  int st = check_lat(c, lat);
+  if (st) return st;
+  if (!out) return fail(MCL_ERR_ARG, "null argument");
+  if ((st = run_measure(c, 0))) return st;
+  ObsDev o;
+  MCL_HIP(hipMemcpyAsync(&o, c->obs_d + lat, sizeof(o), hipMemcpyDeviceToHost, c->stream));
+  MCL_HIP(hipStreamSynchronize(c->stream));
+  out->nvalid = (int64_t)o.nvalid;
+  out->E = o.E;
+  for (int a = 0; a < 6; a++) out->Q[a] = o.Q[a];
+  out->S = o.S;
+  for (int a = 0; a < 3; a++) out->director[a] = o.dir[a];
+  out->Sr_sph = o.srs;
+  out->Pr_sph = o.prs;
+  out->Sr_cyl = o.src;
+  out->Pr_cyl = o.prc;
+  out->P = o.P;
+  out->dtheta = o.dth;
+  out->divn = o.divn;
+  return MCL_OK;
+}
+
+int mcl_get_directors(void *ctx, int32_t lat, float *ni) {
+  Ctx *c = as_ctx(ctx);
+  int st = check_lat(c, lat);
+  if (st) return st;
+  if (!ni) return fail(MCL_ERR_ARG, "null argument");
+  const LatHost &h = c->lh[lat];
+  const size_t n = 2 * (size_t)h.nc;
+  std::vector<float> x(n), y(n), z(n);
+  MCL_HIP(hipMemcpyAsync(x.data(), c->X + h.off, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream));
+  MCL_HIP(hipMemcpyAsync(y.data(), c->Y + h.off, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream));
+  MCL_HIP(hipMemcpyAsync(z.data(), c->Z + h.off, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream));
+  MCL_HIP(hipStreamSynchronize(c->stream));
+  for (int k = 0; k < h.nz; k++)
+    for (int j = 0; j < h.ny; j++)
+      for (int i = 0; i < h.nx; i++) {
+        const size_t lin = (size_t)i + (size_t)h.nx * (j + (size_t)h.ny * k);
+        const size_t t = (size_t)((i + j + k) & 1) * h.nc + (i >> 1) + (size_t)h.nxh * (j + (size_t)h.ny * k);
+        ni[3 * lin] = x[t];
+        ni[3 * lin + 1] = y[t];
+        ni[3 * lin + 2] = z[t];
+      }
+  return MCL_OK;
+}
+
+int mcl_set_directors(void *ctx, int32_t lat, const float *ni) {
+  Ctx *c = as_ctx(ctx);
+  int st = check_lat(c, lat);
+  if (st) return st;
+  if (!ni) return fail(MCL_ERR_ARG, "null argument");
+  return upload_directors(c, lat, ni);
+}
+
+int mcl_local_order(void *ctx, int32_t lat, float *out) {
+  Ctx *c = as_ctx(ctx);
+  int st = check_lat(c, lat);
+  if (st) return st;
+  if (!out) return fail(MCL_ERR_ARG, "null argument");
+  const LatHost &h = c->lh[lat];
+  const int n = 2 * h.nc;
+  const dim3 grid((n + kBlock - 1) / kBlock);
+  if (c->mask)
+    hipLaunchKernelGGL(k_local_order<true>, grid, dim3(kBlock), 0, c->stream, c->lat_d, lat, c->X, c->Y, c->Z,
+                       c->PT, c->scratch_f);
+  else
+    hipLaunchKernelGGL(k_local_order<false>, grid, dim3(kBlock), 0, c->stream, c->lat_d, lat, c->X, c->Y, c->Z,
+                       c->PT, c->scratch_f);
+  MCL_HIP(hipGetLastError());
+  const size_t Nt = (size_t)h.nx * h.ny * h.nz;
+  MCL_HIP(hipMemcpyAsync(out, c->scratch_f, sizeof(float) * Nt, hipMemcpyDeviceToHost, c->stream));
+  MCL_HIP(hipStreamSynchronize(c->stream));
+  return MCL_OK;
+}
+
+int mcl_synchronize(void *ctx) {
+  Ctx *c = as_ctx(ctx);
+  if (!c) return fail(MCL_ERR_ARG, "null context");
+  MCL_HIP(hipStreamSynchronize(c->stream));
+  return MCL_OK;
+}
+
+int mcl_site_energies(void *ctx, int32_t lat, const int32_t *sites, const float *trial, int32_t n, float *e_cur,
+                      float *e_trial) {
+  Ctx *c = as_ctx(ctx);
+  int st = check_lat(c, lat);
+  if (st) return st;
+  if (n <= 0) return MCL_OK;
+  if (!sites || !trial || !e_cur || !e_trial) return fail(MCL_ERR_ARG, "null argument");
+  const LatHost &h = c->lh[lat];
+  const long long Nt = (long long)h.nx * h.ny * h.nz;
+  for (int g = 0; g < n; g++)
+    if (sites[g] < 0 || sites[g] >= Nt) return fail(MCL_ERR_ARG, "site index out of range");
+  int *d_sites = nullptr;
+  float *d_trial = nullptr, *d_ec = nullptr, *d_et = nullptr;
+  if ((st = dev_alloc(&d_sites, n)) || (st = dev_alloc(&d_trial, 3 * (size_t)n)) || (st = dev_alloc(&d_ec, n)) ||
+      (st = dev_alloc(&d_et, n))) {
+    (void)hipFree(d_sites);
+    (void)hipFree(d_trial);
+    (void)hipFree(d_ec);
+    (void)hipFree(d_et);
+    return st;
+  }
+  hipError_t e = hipMemcpyAsync(d_sites, sites, sizeof(int) * n, hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(d_trial, trial, sizeof(float) * 3 * n, hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) {
+    switch (c->pot) {
+      case MCL_POT_GHRL: dispatch_site_energies<MCL_POT_GHRL>(c, lat, d_sites, d_trial, n, d_ec, d_et); break;
+      case MCL_POT_PEAR: dispatch_site_energies<MCL_POT_PEAR>(c, lat, d_sites, d_trial, n, d_ec, d_et); break;
+      default: dispatch_site_energies<MCL_POT_LL>(c, lat, d_sites, d_trial, n, d_ec, d_et); break;
+    }
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(e_cur, d_ec, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(e_trial, d_et, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  (void)hipFree(d_sites);
+  (void)hipFree(d_trial);
+  (void)hipFree(d_ec);
+  (void)hipFree(d_et);
+  if (e != hipSuccess) return fail(MCL_ERR_DEVICE, std::string("site energies: ") + hipGetErrorString(e));
+  return MCL_OK;
+}
+
+int mcl_philox(const uint32_t *ctr, const uint32_t *key, int32_t n, uint32_t *out) {
+  if (n <= 0) return MCL_OK;
+  if (!ctr || !key || !out) return fail(MCL_ERR_ARG, "null argument");
+  uint32_t *d_ctr = nullptr, *d_key = nullptr, *d_out = nullptr;
+  int st;
+  if ((st = dev_alloc(&d_ctr, 4 * (size_t)n)) || (st = dev_alloc(&d_key, 2 * (size_t)n)) ||
+      (st = dev_alloc(&d_out, 4 * (size_t)n))) {
+    (void)hipFree(d_ctr);
+    (void)hipFree(d_key);
+    (void)hipFree(d_out);
+    return st;
+  }
+  hipError_t e = hipMemcpy(d_ctr, ctr, sizeof(uint32_t) * 4 * n, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(d_key, key, sizeof(uint32_t) * 2 * n, hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_philox, dim3((n + 255) / 256), dim3(256), 0, 0, d_ctr, d_key, n, d_out);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpy(out, d_out, sizeof(uint32_t) * 4 * n, hipMemcpyDeviceToHost);
+  (void)hipFree(d_ctr);
+  (void)hipFree(d_key);
+  (void)hipFree(d_out);
+  if (e != hipSuccess) return fail(MCL_ERR_DEVICE, std::string("philox: ") + hipGetErrorString(e));
+  return MCL_OK;
+}
+
+}  // extern "C"

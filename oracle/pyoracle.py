@@ -1,0 +1,291 @@
+"""ctypes binding of the CPU oracle (oracle/mcl_oracle.c) -- TEST INFRASTRUCTURE ONLY.
+
+Imported by tests/, by the smoke check of __graft_entry__ and by the cpu_baseline leg of bench.py, and by nothing
+else. It builds an `orc_params` from keyword arguments and exposes the oracle functions on numpy arrays.
+"""
+import ctypes
+import math
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "build", "libmcloracle.so")
+MAX_SURF = 64
+
+POT = {"ll": 0, "ghrl": 1, "pear": 2}
+GEOM = {"bulk": 0, "slab": 1, "sphere": 2, "custom": 3}
+ANCH = {"homeotropic": 0, "rp": 1, "rp_tangent": 2, "fg": 3, "homeotropic_ghrl": 4, "rp_ghrl": 5, "fg_ghrl": 6}
+
+
+class OrcParams(ctypes.Structure):
+    _fields_ = [("Nx", ctypes.c_int), ("Ny", ctypes.c_int), ("Nz", ctypes.c_int), ("periodic", ctypes.c_int * 3),
+                ("potential", ctypes.c_int), ("A", ctypes.c_float), ("B1", ctypes.c_float), ("C", ctypes.c_float),
+                ("ghrl_lambda", ctypes.c_float), ("ghrl_mu", ctypes.c_float), ("ghrl_nu", ctypes.c_float),
+                ("ghrl_rho", ctypes.c_float), ("ghrl_sigma", ctypes.c_float), ("elecX", ctypes.c_float),
+                ("elecY", ctypes.c_float), ("elecZ", ctypes.c_float), ("elecA", ctypes.c_float),
+                ("elecP", ctypes.c_float), ("elecE", ctypes.c_float), ("flipMove", ctypes.c_int),
+                ("T", ctypes.c_float), ("nSurf", ctypes.c_int), ("surf_kind", ctypes.c_int * MAX_SURF),
+                ("surf_W", ctypes.c_float * MAX_SURF), ("surf_axis", (ctypes.c_float * 3) * MAX_SURF)]
+
+
+class OrcRng(ctypes.Structure):
+    _fields_ = [("s", ctypes.c_uint32 * 4)]
+
+
+F = ctypes.POINTER(ctypes.c_float)
+I = ctypes.POINTER(ctypes.c_int)
+D = ctypes.POINTER(ctypes.c_double)
+P = ctypes.POINTER(OrcParams)
+FA = ctypes.c_float * 3
+
+_SIGS = {
+    "orc_pot_ll": (ctypes.c_float, [P, F, F, F]),
+    "orc_pot_ghrl": (ctypes.c_float, [P, F, F, F]),
+    "orc_pot_pear": (ctypes.c_float, [P, F, F, F]),
+    "orc_pair": (ctypes.c_float, [P, F, F, F]),
+    "orc_field": (ctypes.c_float, [P, F]),
+    "orc_surface": (ctypes.c_float, [P, ctypes.c_int, F, F]),
+    "orc_set_ghrl": (None, [P, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float, F]),
+    "orc_lab_axis": (None, [ctypes.c_float, ctypes.c_float, F]),
+    "orc_site_energy": (ctypes.c_float, [P, F, I, F, ctypes.c_int, F]),
+    "orc_energy_mean": (ctypes.c_double, [P, F, I, F]),
+    "orc_energy_total": (ctypes.c_double, [P, F, I, F]),
+    "orc_geometry": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, I, F]),
+    "orc_ic_homogeneous": (None, [ctypes.c_int, I, F, ctypes.c_float, ctypes.c_float]),
+    "orc_ic_random": (None, [ctypes.c_int, I, F, ctypes.c_uint64]),
+    "orc_count_valid": (ctypes.c_int, [ctypes.c_int, I]),
+    "orc_gpu_surface_vectors": (ctypes.c_int, [P, ctypes.c_int, I, F, F]),
+    "orc_rng_seed": (None, [ctypes.POINTER(OrcRng), ctypes.c_uint64, ctypes.c_uint64]),
+    "orc_rng_uniform": (ctypes.c_float, [ctypes.POINTER(OrcRng)]),
+    "orc_sweep": (ctypes.c_long, [P, F, I, F, F, ctypes.POINTER(OrcRng), ctypes.c_int]),
+    "orc_ang_var_update": (ctypes.c_float, [ctypes.c_float, ctypes.c_long, ctypes.c_int]),
+    "orc_rotate": (None, [F, ctypes.c_float, ctypes.c_float, F]),
+    "orc_q_tensor": (None, [ctypes.c_int, F, I, D]),
+    "orc_eig_sym3": (ctypes.c_double, [D, D, D]),
+    "orc_radial_order": (None, [P, F, I, ctypes.c_int, D, D]),
+    "orc_local_order": (ctypes.c_double, [P, F, I, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    "orc_polarization": (ctypes.c_double, [ctypes.c_int, F, I, D]),
+    "orc_splay_magnitude": (ctypes.c_double, [P, F, I]),
+    "orc_splay_divergence": (ctypes.c_double, [P, F, I]),
+    "orc_thermal_run": (ctypes.c_int, [P, F, I, F, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                       ctypes.c_int, ctypes.c_uint64, ctypes.c_int, D, ctypes.c_int]),
+    "orc_time_sweeps": (ctypes.c_double, [P, F, I, F, ctypes.c_int, ctypes.c_uint64, ctypes.c_int, F]),
+    "orc_sweep_points": (ctypes.c_int, [ctypes.c_double, ctypes.c_double, ctypes.c_double]),
+    "orc_sweep_value": (ctypes.c_double, [ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                          ctypes.c_int]),
+    "orc_snapshot_scale": (ctypes.c_int, [ctypes.c_double]),
+    "orc_snapshot_index": (ctypes.c_long, [ctypes.c_double, ctypes.c_int]),
+    "orc_philox4x32_10": (None, [ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32),
+                                 ctypes.POINTER(ctypes.c_uint32)]),
+    "orc_max_threads": (ctypes.c_int, []),
+}
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("oracle not built: make -C oracle (or python -m mclics_amd.build oracle)")
+        l = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(l, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = l
+    return _lib
+
+
+def fp(a):
+    return a.ctypes.data_as(F)
+
+
+def ip(a):
+    return a.ctypes.data_as(I)
+
+
+def dp(a):
+    return a.ctypes.data_as(D)
+
+
+def vec(v):
+    return FA(*[float(x) for x in v])
+
+
+def params(Nx=8, Ny=8, Nz=8, periodic=(1, 1, 1), potential="ll", A=1.0, B1=0.04, C=0.3, ghrl=None, k=None,
+           elec=(0.0, 0.0, 0.0), elecA=0.0, elecP=0.0, elecE=1.0, flipMove=None, T=1.0, surfaces=()):
+    """orc_params from keywords. ghrl = (lambda, mu, nu, rho, sigma) or k = (k11, k22, k33, p0) through setGHRL.
+    surfaces: list of (kind, W, axis). The field direction is normalized as in main.cpp:97-106."""
+    p = OrcParams()
+    p.Nx, p.Ny, p.Nz = Nx, Ny, Nz
+    p.periodic[:] = list(periodic)
+    p.potential = POT[potential] if isinstance(potential, str) else potential
+    p.A, p.B1, p.C = A, B1, C
+    if k is not None:
+        lib().orc_set_ghrl(ctypes.byref(p), *[float(x) for x in k], None)
+    elif ghrl is not None:
+        p.ghrl_lambda, p.ghrl_mu, p.ghrl_nu, p.ghrl_rho, p.ghrl_sigma = ghrl
+    else:
+        p.ghrl_nu = 1.0
+    if elecA != 0 or elecP != 0:
+        n = math.sqrt(sum(x * x for x in elec))
+        elec = tuple(np.float32(x) / np.float32(n) for x in elec)
+    p.elecX, p.elecY, p.elecZ = elec
+    p.elecA, p.elecP, p.elecE = elecA, elecP, elecE
+    p.flipMove = (1 if (p.potential == 2 or elecP != 0) else 0) if flipMove is None else flipMove
+    p.T = T
+    p.nSurf = len(surfaces)
+    for i, (kind, W, axis) in enumerate(surfaces):
+        p.surf_kind[i] = ANCH[kind] if isinstance(kind, str) else kind
+        p.surf_W[i] = W
+        for a in range(3):
+            p.surf_axis[i][a] = axis[a]
+    return p
+
+
+def ghrl_coeffs(k11, k22, k33, p0=0.0):
+    p = OrcParams()
+    s = ctypes.c_float()
+    lib().orc_set_ghrl(ctypes.byref(p), k11, k22, k33, p0, ctypes.byref(s))
+    return dict(lam=p.ghrl_lambda, mu=p.ghrl_mu, nu=p.ghrl_nu, rho=p.ghrl_rho, sigma=p.ghrl_sigma, scale=s.value)
+
+
+def lab_axis(theta, phi):
+    out = (ctypes.c_float * 3)()
+    lib().orc_lab_axis(theta, phi, out)
+    return tuple(out)
+
+
+def pair(p, ni, nj, rij):
+    return lib().orc_pair(ctypes.byref(p), vec(ni), vec(nj), vec(rij))
+
+
+def geometry(name, Nx, Ny, Nz):
+    nt = Nx * Ny * Nz
+    pt = np.zeros(nt, dtype=np.int32)
+    ns = np.zeros(3 * nt, dtype=np.float32)
+    lib().orc_geometry(GEOM[name], Nx, Ny, Nz, ip(pt), fp(ns))
+    return pt, ns
+
+
+def ic_homogeneous(pt, theta=0.0, phi=0.0):
+    ni = np.zeros(3 * len(pt), dtype=np.float32)
+    lib().orc_ic_homogeneous(len(pt), ip(pt), fp(ni), theta, phi)
+    return ni
+
+
+def ic_random(pt, seed):
+    ni = np.zeros(3 * len(pt), dtype=np.float32)
+    lib().orc_ic_random(len(pt), ip(pt), fp(ni), seed)
+    return ni
+
+
+def site_energy(p, ni, pt, ns, idx, n0):
+    return lib().orc_site_energy(ctypes.byref(p), fp(ni), ip(pt), fp(ns) if ns is not None else None, int(idx),
+                                 vec(n0))
+
+
+def energy_mean(p, ni, pt, ns=None):
+    return lib().orc_energy_mean(ctypes.byref(p), fp(ni), ip(pt), fp(ns) if ns is not None else None)
+
+
+def energy_total(p, ni, pt, ns=None):
+    return lib().orc_energy_total(ctypes.byref(p), fp(ni), ip(pt), fp(ns) if ns is not None else None)
+
+
+def gpu_surface_vectors(p, pt, ns):
+    sv = np.zeros_like(ns)
+    n = lib().orc_gpu_surface_vectors(ctypes.byref(p), len(pt), ip(pt), fp(ns), fp(sv))
+    return sv, n
+
+
+def q_tensor(ni, pt):
+    Q = np.zeros(9)
+    lib().orc_q_tensor(len(pt), fp(ni), ip(pt), dp(Q))
+    return Q.reshape(3, 3)
+
+
+def eig_sym3(A):
+    A = np.ascontiguousarray(A, dtype=np.float64).reshape(9)
+    v = np.zeros(3)
+    ev = np.zeros(3)
+    lam = lib().orc_eig_sym3(dp(A), dp(v), dp(ev))
+    return lam, v, ev
+
+
+def radial_order(p, ni, pt, cylinder=False):
+    s, q = ctypes.c_double(), ctypes.c_double()
+    lib().orc_radial_order(ctypes.byref(p), fp(ni), ip(pt), int(cylinder), ctypes.byref(s), ctypes.byref(q))
+    return s.value, q.value
+
+
+def local_order(p, ni, pt, i, j, k):
+    return lib().orc_local_order(ctypes.byref(p), fp(ni), ip(pt), i, j, k)
+
+
+def polarization(ni, pt, director):
+    d = np.ascontiguousarray(director, dtype=np.float64)
+    return lib().orc_polarization(len(pt), fp(ni), ip(pt), dp(d))
+
+
+def splay_magnitude(p, ni, pt):
+    return lib().orc_splay_magnitude(ctypes.byref(p), fp(ni), ip(pt))
+
+
+def splay_divergence(p, ni, pt):
+    return lib().orc_splay_divergence(ctypes.byref(p), fp(ni), ip(pt))
+
+
+def make_rngs(n, seed):
+    rngs = (OrcRng * n)()
+    for t in range(n):
+        lib().orc_rng_seed(ctypes.byref(rngs[t]), seed * 104729 + t, 0x5EED)
+    return rngs
+
+
+def sweep(p, ni, pt, ns, ang_var, rngs, nthreads):
+    a = ctypes.c_float(ang_var)
+    acc = lib().orc_sweep(ctypes.byref(p), fp(ni), ip(pt), fp(ns) if ns is not None else None, ctypes.byref(a),
+                          rngs, nthreads)
+    return acc, a.value
+
+
+def thermal_run(p, ni, pt, ns, Ti, Tf, dT, MCT, MCS, seed, nthreads=1):
+    n = lib().orc_sweep_points(Ti, Tf, dT)
+    rows = np.zeros(6 * n)
+    got = lib().orc_thermal_run(ctypes.byref(p), fp(ni), ip(pt), fp(ns) if ns is not None else None, Ti, Tf, dT,
+                                MCT, MCS, seed, nthreads, dp(rows), n)
+    return rows[:6 * got].reshape(got, 6)
+
+
+def time_sweeps(p, ni, pt, ns, nsweeps, seed, nthreads, ang_var=0.5):
+    a = ctypes.c_float(ang_var)
+    t = lib().orc_time_sweeps(ctypes.byref(p), fp(ni), ip(pt), fp(ns) if ns is not None else None, nsweeps, seed,
+                              nthreads, ctypes.byref(a))
+    return t, a.value
+
+
+def philox(ctr, key):
+    c = (ctypes.c_uint32 * 4)(*ctr)
+    k = (ctypes.c_uint32 * 2)(*key)
+    o = (ctypes.c_uint32 * 4)()
+    lib().orc_philox4x32_10(c, k, o)
+    return list(o)
+
+
+def sweep_points(a, b, d):
+    return lib().orc_sweep_points(a, b, d)
+
+
+def sweep_value(a, b, d, n, npoints):
+    return lib().orc_sweep_value(a, b, d, n, npoints)
+
+
+def snapshot_scale(step):
+    return lib().orc_snapshot_scale(step)
+
+
+def snapshot_index(x, S):
+    return lib().orc_snapshot_index(x, S)
