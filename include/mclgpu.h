@@ -161,9 +161,10 @@ int mcl_site_energies(void *ctx, int32_t lat, const int32_t *sites, const float 
 int mcl_philox(const uint32_t *ctr, const uint32_t *key, int32_t n, uint32_t *out);
 
 /* ---- timing (used by the benchmark) ---- */
-/* When stride > 0, mcl_sweeps records HIP events around each of the two sweep-kernel launches of every
- * stride-th sweep (events for max_timed_sweeps timed sweeps are created up front); mcl_kernel_timing returns the
- * number of timed launches and their summed duration (synchronizes). stride = 0 switches timing off. */
+/* When stride > 0, mcl_sweeps records a HIP event pair around each run of consecutive sweep-kernel launches (at
+ * most stride sweeps per run; a measurement ends a run), using events created up front for max_timed_sweeps
+ * sweeps. mcl_kernel_timing returns the number of timed sweep-kernel launches and their summed duration
+ * (synchronizes). stride = 0 switches timing off. */
 int mcl_set_kernel_timing(void *ctx, int32_t stride, int32_t max_timed_sweeps);
 int mcl_kernel_timing(void *ctx, int64_t *launches, double *total_ms);
 /* Device stream of the context (a hipStream_t), for callers that record their own events. */

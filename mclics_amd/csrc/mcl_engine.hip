@@ -12,14 +12,16 @@
 // the active colour (proposal of evolve.cu:162-182, energies of geometry.cu:25-120 / potential.cu:11-58, rule of
 // evolve.cu:212-239). Random numbers come from Philox4x32-10 keyed by (seed, lattice) with counter
 // (site, sweep), so the generator has no state in memory. Accepted rotations are counted with a wave ballot and
-// stored per workgroup; a small close kernel (one workgroup per lattice) sums them, applies the ang_var rule of
-// evolve.cu:430-436 on the device and advances the sweep counter, so no host round trip is needed per sweep.
+// one atomic per workgroup into 64 striped counters per lattice (a ring of three sweeps); the colour-0 pass of the
+// next sweep sums the 64 counters in every wave and applies the ang_var rule of evolve.cu:430-436 on the device,
+// so the rule is exact and needs neither a host round trip nor an extra launch per sweep.
 //
 // A measurement is one pass over both colours that accumulates, in FP64, the site energy (get_Energy,
 // evolve.cu:66-108), the Q-tensor sums (parameter_order.cpp:40-71), the radial order (parameter_order.cpp:223-245)
 // and the pear observables (parameter_order.cpp:131-215) into one partial per workgroup; a finalize kernel sums the
 // partials of each lattice in a fixed order (bitwise reproducible), solves the 3x3 eigenproblem for S in FP64 and
 // adds the sample to the lattice's FP64 moments.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -38,7 +40,7 @@ namespace {
 
 constexpr int kBlock = 256;          // threads per workgroup (4 wave64)
 constexpr int kWaves = kBlock / 64;
-constexpr int kMeasPerThread = 4;    // sites per thread in a measurement workgroup
+constexpr int kMeasPerThread = 8;    // sites per thread in a measurement workgroup
 constexpr int kNQ = 24;              // FP64 quantities per measurement partial
 constexpr uint32_t kTagSweep = 0x4D435357u;  // counter word 3 of the sweep draws
 
@@ -77,11 +79,20 @@ struct LatDev {
   float cx, cy, cz;  // centre of the radial observables (Nx/2, Ny/2, Nz/2 with integer division)
 };
 
+// Per-lattice control block. av_store[p] is the ang_var used by the sweeps of parity p (written by the lattice's
+// first colour-0 workgroup); a value set by the host (av_set) replaces the rule at sweep av_set_sweep.
 struct CtrlDev {
-  float T, elecE, ang_var;
-  unsigned int last_acc;  // accepted rotations of the last completed sweep
-  unsigned long long sweep;
+  float T, invT, elecE, av_set;
+  float av_store[2];
+  unsigned long long av_set_sweep;
 };
+
+// Striped acceptance counters: kCnt per lattice and sweep, each in its own 128-byte cache line (same-address or
+// same-line device-scope atomics from thousands of workgroups serialize at about 12 ns each).
+constexpr int kCnt = 64;
+constexpr int kCntStride = 32;                     // uint32 per counter slot (128 bytes)
+constexpr int kCntSweep = kCnt * kCntStride;       // one sweep's counters
+constexpr int kCntLattice = 3 * kCntSweep;         // ring of three sweeps
 
 // Device anchoring kernels (the GPU selection of evolve.cu:280-297): 0 = -W (n.v)^2 (homeotropic, rp, rp_tangent,
 // with v the per-site vector), 1 = +W (n.v)^2 (fg), 2 = W U_GHRL(n, v, v), 3 = W U_GHRL(n, axis, v),
@@ -105,6 +116,17 @@ __device__ __forceinline__ float dot3(V3 a, V3 b) { return a.x * b.x + a.y * b.y
 template <int AX>
 __device__ __forceinline__ float comp(V3 a) {
   return AX == 0 ? a.x : (AX == 1 ? a.y : a.z);
+}
+
+// Loads and stores at a 32-bit element index of a uniform base pointer. The pool of a context holds fewer than
+// 2^30 sites, so the byte offset fits in 32 bits and the compiler can use the scalar-base + 32-bit-offset form.
+template <typename T>
+__device__ __forceinline__ T ldx(const T *__restrict__ base, uint32_t idx) {
+  return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + (idx * (uint32_t)sizeof(T)));
+}
+template <typename T>
+__device__ __forceinline__ void stx(T *__restrict__ base, uint32_t idx, T v) {
+  *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + (idx * (uint32_t)sizeof(T))) = v;
 }
 
 // ---- pair potentials (potential.cpp:19-56 / potential.cu:11-43) ----
@@ -230,32 +252,21 @@ __device__ __forceinline__ bool gather(const LatDev &L, int colour, int t, const
   self = base + (uint32_t)t;
   s.ptv = 1;
   if constexpr (MASK) {
-    s.ptv = PT[self];
+    s.ptv = ldx(PT, self);
     if (s.ptv == 0) return false;
   }
-  s.n = V3{X[self], Y[self], Z[self]};
-  // Periodic_Boundary / Free_Boundary (potential.cu:50-58)
+  s.n = V3{ldx(X, self), ldx(Y, self), ldx(Z, self)};
+  // Periodic_Boundary / Free_Boundary (potential.cu:50-58). A missing neighbour keeps an in-bounds index (the
+  // site's own column in the other colour) and is masked out of the energy, so all loads are unconditional.
   const bool px = L.flags & F_PX, py = L.flags & F_PY, pz = L.flags & F_PZ;
-  int ip = i + 1, im = i - 1, jp = j + 1, jm = j - 1, kp = k + 1, km = k - 1;
   unsigned ex = 63u;
-  if (ip == nx) {
-    if (px) ip = 0; else ex &= ~1u;
-  }
-  if (im < 0) {
-    if (px) im = nx - 1; else ex &= ~2u;
-  }
-  if (jp == ny) {
-    if (py) jp = 0; else ex &= ~4u;
-  }
-  if (jm < 0) {
-    if (py) jm = ny - 1; else ex &= ~8u;
-  }
-  if (kp == nz) {
-    if (pz) kp = 0; else ex &= ~16u;
-  }
-  if (km < 0) {
-    if (pz) km = nz - 1; else ex &= ~32u;
-  }
+  int ip = i + 1, im = i - 1, jp = j + 1, jm = j - 1, kp = k + 1, km = k - 1;
+  if (ip == nx) { ip = px ? 0 : i; ex &= px ? 63u : ~1u; }
+  if (im < 0) { im = px ? nx - 1 : i; ex &= px ? 63u : ~2u; }
+  if (jp == ny) { jp = py ? 0 : j; ex &= py ? 63u : ~4u; }
+  if (jm < 0) { jm = py ? ny - 1 : j; ex &= py ? 63u : ~8u; }
+  if (kp == nz) { kp = pz ? 0 : k; ex &= pz ? 63u : ~16u; }
+  if (km < 0) { km = pz ? nz - 1 : k; ex &= pz ? 63u : ~32u; }
   const uint32_t rowoff = obase + (uint32_t)(nxh * (j + ny * k));
   uint32_t nidx[6];
   nidx[0] = rowoff + (uint32_t)(ip >> 1);
@@ -266,32 +277,31 @@ __device__ __forceinline__ bool gather(const LatDev &L, int colour, int t, const
   nidx[5] = obase + (uint32_t)(m + nxh * (j + ny * km));
 #pragma unroll
   for (int d = 0; d < 6; d++) {
-    if (ex & (1u << d)) {
-      const uint32_t q = nidx[d];
-      s.nb[d] = V3{X[q], Y[q], Z[q]};
-      if constexpr (MASK) {
-        if (PT[q] == 0) ex &= ~(1u << d);
-      }
-    } else {
-      s.nb[d] = V3{0.0f, 0.0f, 1.0f};
+    const uint32_t q = nidx[d];
+    s.nb[d] = V3{ldx(X, q), ldx(Y, q), ldx(Z, q)};
+    if constexpr (MASK) {
+      if (ldx(PT, q) == 0) ex &= ~(1u << d);
     }
   }
   s.ex = ex;
   s.sv = V3{0.0f, 0.0f, 0.0f};
   if constexpr (SURF) {
-    if (s.ptv > 1) s.sv = V3{SX[self], SY[self], SZ[self]};
+    if (s.ptv > 1) s.sv = V3{ldx(SX, self), ldx(SY, self), ldx(SZ, self)};
   }
   return true;
 }
 
-// ang_var rule of evolve.cu:430-436 (float ang_var, double comparisons as in the reference).
-__device__ __forceinline__ float ang_var_rule(float av, unsigned acc, int nvalid) {
+// ang_var rule of evolve.cu:430-436. The reference compares the double quotient acc / nvalid with 0.5; for
+// integers below 2^31 that is the same as comparing 2 acc with nvalid (the quotient is never within rounding of
+// 0.5 unless it equals 0.5), which avoids an FP64 division. The updates keep the reference's float * double
+// arithmetic.
+__host__ __device__ __forceinline__ float ang_var_rule(float av, unsigned acc, int nvalid) {
   if (nvalid <= 0) return av;
-  const double frac = (double)acc / (double)nvalid;
-  if (frac < 0.5) {
+  const long long twice = 2ll * (long long)acc;
+  if (twice < (long long)nvalid) {
     av = (float)((double)av * 0.99);
     if ((double)av < 0.01) av = 0.01f;
-  } else if (frac > 0.5) {
+  } else if (twice > (long long)nvalid) {
     av = (float)((double)av / 0.99);
     if ((double)av > 1.0) av = (float)((double)av - 0.5);
   }
@@ -303,9 +313,9 @@ __device__ __forceinline__ float ang_var_rule(float av, unsigned acc, int nvalid
 // workgroup L; consecutive logical workgroups are placed on the same XCD (workgroups are dispatched to the 8 XCDs
 // round robin), so the rows a workgroup shares with its neighbours in z and y mostly hit the same L2.
 template <int POT, bool MASK, bool SURF, bool FIELD, bool FLIP>
-__global__ __launch_bounds__(kBlock) void k_sweep(const LatDev *__restrict__ lats, const CtrlDev *__restrict__ ctrl,
-                                                  unsigned int *__restrict__ bcount, const int2 *__restrict__ blk,
-                                                  int nblk, int chunk, int colour,
+__global__ __launch_bounds__(kBlock) void k_sweep(const LatDev *__restrict__ lats, CtrlDev *__restrict__ ctrl,
+                                                  unsigned int *__restrict__ cnt, const int2 *__restrict__ blk,
+                                                  int nblk, int chunk, int colour, unsigned long long sweep,
                                                   float *__restrict__ X, float *__restrict__ Y,
                                                   float *__restrict__ Z, const uint8_t *__restrict__ PT,
                                                   const float *__restrict__ SX, const float *__restrict__ SY,
@@ -316,8 +326,33 @@ __global__ __launch_bounds__(kBlock) void k_sweep(const LatDev *__restrict__ lat
   const int2 bd = blk[Lb];
   const int r = bd.x;
   const LatDev &L = lats[r];
-  const float T = ctrl[r].T, av = ctrl[r].ang_var, elecE = ctrl[r].elecE;
-  const unsigned long long sweep = ctrl[r].sweep;
+  const float T = ctrl[r].T, invT = ctrl[r].invT, elecE = ctrl[r].elecE;
+  const int lane = threadIdx.x & 63;
+  const int lblk = bd.y / kBlock;  // workgroup index within the lattice
+  unsigned int *const cnt_r = cnt + (size_t)r * kCntLattice;
+  __shared__ float av_sh;
+  float av;
+  if (colour == 1) {
+    av = ctrl[r].av_store[sweep & 1];
+  } else if (sweep == ctrl[r].av_set_sweep) {
+    av = ctrl[r].av_set;
+  } else {
+    // accepted rotations of the previous sweep: wave 0 sums its 64 counters and shares the ang_var rule's result
+    if (threadIdx.x < 64) {
+      unsigned a = cnt_r[((sweep - 1) % 3) * kCntSweep + lane * kCntStride];
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o, 64);
+      if (threadIdx.x == 0) av_sh = ang_var_rule(ctrl[r].av_store[(sweep - 1) & 1], a, L.nvalid);
+    }
+    __syncthreads();
+    av = av_sh;
+  }
+  if (colour == 0 && lblk == 0 && threadIdx.x < kCnt) {
+    // the lattice's first workgroup publishes ang_var for colour 1 and the next sweep, and clears the counters
+    // of sweep + 1 (last read by the colour-0 pass of sweep - 1)
+    if (threadIdx.x == 0) ctrl[r].av_store[sweep & 1] = av;
+    cnt_r[((sweep + 1) % 3) * kCntSweep + threadIdx.x * kCntStride] = 0u;
+  }
   const int t = bd.y + (int)threadIdx.x;
 
   bool accepted = false;
@@ -330,8 +365,8 @@ __global__ __launch_bounds__(kBlock) void k_sweep(const LatDev *__restrict__ lat
     // trial rotation by pi * va about x, y or z (evolve.cu:162-182)
     const float va = (2.0f * mcl::u01_open(R.v[0]) - 1.0f) * av;
     const float rot = mcl::u01_open(R.v[1]);
-    float sn, cs;
-    sincospif(va, &sn, &cs);
+    // cos and sin of pi va: the hardware sine and cosine take their argument in turns (x 2 pi)
+    const float sn = __builtin_amdgcn_sinf(0.5f * va), cs = __builtin_amdgcn_cosf(0.5f * va);
     const V3 n = s.n;
     V3 nn;
     if (rot < 0.333f) {
@@ -341,7 +376,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep(const LatDev *__restrict__ lat
     } else {
       nn = V3{n.x * cs + n.y * sn, n.y * cs - n.x * sn, n.z};
     }
-    const float inv = 1.0f / sqrtf(dot3(nn, nn));
+    const float inv = __builtin_amdgcn_rsqf(dot3(nn, nn));  // |nn| ~ 1: no denormal handling needed
     nn.x *= inv;
     nn.y *= inv;
     nn.z *= inv;
@@ -349,7 +384,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep(const LatDev *__restrict__ lat
     const float E_new = site_energy<POT, SURF, FIELD>(L, s, nn, surf, elecE);
     const float u = mcl::u01_open(R.v[2]);
     // Metropolis; T = 0 accepts when the energy does not increase (evolve.cu:212-224)
-    accepted = T > 0.0f ? (u <= __expf(-(E_new - E_old) / T)) : (E_new <= E_old);
+    accepted = T > 0.0f ? (u <= __expf(-(E_new - E_old) * invT)) : (E_new <= E_old);
     V3 nc = accepted ? nn : n;
     bool changed = accepted;
     if constexpr (FLIP) {
@@ -358,20 +393,20 @@ __global__ __launch_bounds__(kBlock) void k_sweep(const LatDev *__restrict__ lat
       const V3 nf{-nc.x, -nc.y, -nc.z};
       const float E_f = site_energy<POT, SURF, FIELD>(L, s, nf, surf, elecE);
       const float uf = mcl::u01_open(R.v[3]);
-      if (T > 0.0f ? (uf <= __expf(-(E_f - E_cur) / T)) : (E_f <= E_cur)) {
+      if (T > 0.0f ? (uf <= __expf(-(E_f - E_cur) * invT)) : (E_f <= E_cur)) {
         nc = nf;
         changed = true;
       }
     }
     if (changed) {
-      X[self] = nc.x;
-      Y[self] = nc.y;
-      Z[self] = nc.z;
+      stx(X, self, nc.x);
+      stx(Y, self, nc.y);
+      stx(Z, self, nc.z);
     }
   }
 
-  // accepted rotations: wave ballot, then one plain store per workgroup into its own slot (no atomics: same-address
-  // device-scope atomics from thousands of workgroups on 8 XCDs serialize)
+  // accepted rotations: wave ballot, then one atomic per workgroup into one of kCnt counters (striping keeps the
+  // device-scope atomics of thousands of workgroups from serializing on one address)
   __shared__ unsigned int wsum[kWaves];
   const unsigned long long ball = __ballot(accepted ? 1 : 0);
   if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = (unsigned)__popcll(ball);
@@ -380,31 +415,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep(const LatDev *__restrict__ lat
     unsigned tot = 0;
 #pragma unroll
     for (int w = 0; w < kWaves; w++) tot += wsum[w];
-    bcount[colour * nblk + Lb] = tot;
-  }
-}
-
-// Closes a sweep (one workgroup per lattice): sums the accepted counts of the lattice's workgroups of both colour
-// passes in a fixed order, applies the ang_var rule of evolve.cu:430-436 and advances the sweep counter.
-__global__ __launch_bounds__(kBlock) void k_close_sweep(const LatDev *__restrict__ lats, CtrlDev *__restrict__ ctrl,
-                                                        const unsigned int *__restrict__ bcount, int nblk) {
-  const int r = blockIdx.x;
-  const LatDev &L = lats[r];
-  unsigned v = 0;
-  for (int b = (int)threadIdx.x; b < L.nblk; b += kBlock) v += bcount[L.blk0 + b] + bcount[nblk + L.blk0 + b];
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  __shared__ unsigned int ws[kWaves];
-  if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = v;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    unsigned a = 0;
-    for (int w = 0; w < kWaves; w++) a += ws[w];
-    CtrlDev c = ctrl[r];
-    c.ang_var = ang_var_rule(c.ang_var, a, L.nvalid);
-    c.last_acc = a;
-    c.sweep += 1ull;
-    ctrl[r] = c;
+    if (tot) atomicAdd(&cnt_r[(sweep % 3) * kCntSweep + (lblk & (kCnt - 1)) * kCntStride], tot);
   }
 }
 
@@ -418,7 +429,10 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-template <int POT, bool MASK, bool SURF, bool FIELD, bool PEAR>
+// The basic quantities (energy, Q sums, count) are 0..6 and 21; XQ adds the radial, polar and pear ones.
+__host__ __device__ constexpr bool basic_q(int a) { return a <= 6 || a == 21; }
+
+template <int POT, bool MASK, bool SURF, bool FIELD, bool XQ>
 __global__ __launch_bounds__(kBlock) void k_measure(const LatDev *__restrict__ lats, const CtrlDev *__restrict__ ctrl,
                                                     const int4 *__restrict__ mblk, int nmblk,
                                                     const float *__restrict__ X, const float *__restrict__ Y,
@@ -451,7 +465,7 @@ __global__ __launch_bounds__(kBlock) void k_measure(const LatDev *__restrict__ l
     q[5] += y * z;
     q[21] += z * z;
     q[6] += 1.0;
-    if (L.flags & (F_RSPH | F_RCYL)) {
+    if (XQ && (L.flags & (F_RSPH | F_RCYL))) {
       const float rx = s.i - L.cx, ry = s.j - L.cy, rz = s.k - L.cz;
       if (L.flags & F_RSPH) {
         const float rr = sqrtf(rx * rx + ry * ry + rz * rz);
@@ -472,7 +486,7 @@ __global__ __launch_bounds__(kBlock) void k_measure(const LatDev *__restrict__ l
         }
       }
     }
-    if constexpr (PEAR) {
+    if constexpr (XQ && POT == MCL_POT_PEAR) {
       q[13] += x;
       q[14] += y;
       q[15] += z;
@@ -511,7 +525,7 @@ __global__ __launch_bounds__(kBlock) void k_measure(const LatDev *__restrict__ l
         if (nbc) div += dd / nbc;
       }
       q[20] += div;
-    } else {
+    } else if constexpr (XQ) {
       q[13] += x;
       q[14] += y;
       q[15] += z;
@@ -521,11 +535,12 @@ __global__ __launch_bounds__(kBlock) void k_measure(const LatDev *__restrict__ l
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
   for (int a = 0; a < kNQ; a++) {
+    if (!XQ && !basic_q(a)) continue;
     const double v = wave_sum(q[a]);
     if (lane == 0) red[w][a] = v;
   }
   __syncthreads();
-  if (threadIdx.x < kNQ) {
+  if (threadIdx.x < kNQ && (XQ || basic_q(threadIdx.x))) {
     double v = 0.0;
 #pragma unroll
     for (int ww = 0; ww < kWaves; ww++) v += red[ww][threadIdx.x];
@@ -580,7 +595,7 @@ __device__ double eig_sym3_max(const double a[6], double v[3]) {
 // Sums the partials of each lattice in a fixed order (one workgroup per lattice), computes the observables and,
 // when accumulate != 0, adds the sample to the moments.
 __global__ __launch_bounds__(kBlock) void k_finalize(const LatDev *__restrict__ lats,
-                                                     const double *__restrict__ partials, int nmblk,
+                                                     const double *__restrict__ partials, int nmblk, int xq,
                                                      MomDev *__restrict__ mom, ObsDev *__restrict__ obs,
                                                      int accumulate) {
   const int r = blockIdx.x;
@@ -592,7 +607,8 @@ __global__ __launch_bounds__(kBlock) void k_finalize(const LatDev *__restrict__ 
   for (int a = 0; a < kNQ; a++) v[a] = 0.0;
   for (int p = L.mp0 + (int)threadIdx.x; p < L.mp1; p += kBlock) {
 #pragma unroll
-    for (int a = 0; a < kNQ; a++) v[a] += partials[(size_t)a * nmblk + p];
+    for (int a = 0; a < kNQ; a++)
+      if (xq || basic_q(a)) v[a] += partials[(size_t)a * nmblk + p];
   }
 #pragma unroll
   for (int a = 0; a < kNQ; a++) {
@@ -769,6 +785,7 @@ struct Ctx {
   int nlat = 0;
   int pot = 0;
   bool mask = false, surf = false, field = false, flip = false;
+  bool xq = false;  // radial or pear observables are measured
   std::vector<LatHost> lh;
   std::vector<LatDev> ld;
   long long pool = 0;  // sites in the pooled colour blocks
@@ -779,7 +796,8 @@ struct Ctx {
   LatDev *lat_d = nullptr;
   CtrlDev *ctrl_d = nullptr;
   int2 *blk_d = nullptr;
-  unsigned int *bcount = nullptr;  // accepted rotations per sweep workgroup, [colour][workgroup]
+  unsigned int *cnt = nullptr;  // striped acceptance counters [lattice][sweep % 3][kCnt]
+  unsigned long long sweeps_done = 0;
   int nblk = 0, chunk = 0;
   int4 *mblk_d = nullptr;
   int nmblk = 0;
@@ -792,21 +810,28 @@ struct Ctx {
   int timing_stride = 0;
   std::vector<hipEvent_t> ev;
   size_t ev_used = 0;
-  std::vector<std::pair<size_t, size_t>> ev_pairs;
+  std::vector<std::pair<size_t, int>> ev_runs;  // (first event of the pair, kernel launches in the run)
 };
 
-using SweepFn = void (*)(dim3, dim3, hipStream_t, const Ctx &, int);
+using SweepFn = void (*)(dim3, dim3, hipStream_t, const Ctx &, int, unsigned long long, hipEvent_t, hipEvent_t);
 using MeasFn = void (*)(dim3, dim3, hipStream_t, const Ctx &);
 
+// Launched through hipExtLaunchKernelGGL so that timed launches get start/stop events from the dispatch itself
+// (no extra queue packets between kernels).
 template <int POT, bool MASK, bool SURF, bool FIELD, bool FLIP>
-void launch_sweep(dim3 g, dim3 b, hipStream_t st, const Ctx &c, int colour) {
-  hipLaunchKernelGGL((k_sweep<POT, MASK, SURF, FIELD, FLIP>), g, b, 0, st, c.lat_d, c.ctrl_d, c.bcount, c.blk_d,
-                     c.nblk, c.chunk, colour, c.X, c.Y, c.Z, c.PT, c.SX, c.SY, c.SZ, c.surf_d);
+void launch_sweep(dim3 g, dim3 b, hipStream_t st, const Ctx &c, int colour, unsigned long long sweep, hipEvent_t e0,
+                  hipEvent_t e1) {
+  hipExtLaunchKernelGGL((k_sweep<POT, MASK, SURF, FIELD, FLIP>), g, b, 0, st, e0, e1, 0, c.lat_d, c.ctrl_d, c.cnt,
+                        c.blk_d, c.nblk, c.chunk, colour, sweep, c.X, c.Y, c.Z, c.PT, c.SX, c.SY, c.SZ, c.surf_d);
 }
 template <int POT, bool MASK, bool SURF, bool FIELD>
 void launch_measure(dim3 g, dim3 b, hipStream_t st, const Ctx &c) {
-  hipLaunchKernelGGL((k_measure<POT, MASK, SURF, FIELD, POT == MCL_POT_PEAR>), g, b, 0, st, c.lat_d, c.ctrl_d,
-                     c.mblk_d, c.nmblk, c.X, c.Y, c.Z, c.PT, c.SX, c.SY, c.SZ, c.surf_d, c.partials);
+  if (c.xq)
+    hipLaunchKernelGGL((k_measure<POT, MASK, SURF, FIELD, true>), g, b, 0, st, c.lat_d, c.ctrl_d, c.mblk_d,
+                       c.nmblk, c.X, c.Y, c.Z, c.PT, c.SX, c.SY, c.SZ, c.surf_d, c.partials);
+  else
+    hipLaunchKernelGGL((k_measure<POT, MASK, SURF, FIELD, false>), g, b, 0, st, c.lat_d, c.ctrl_d, c.mblk_d,
+                       c.nmblk, c.X, c.Y, c.Z, c.PT, c.SX, c.SY, c.SZ, c.surf_d, c.partials);
 }
 
 template <int POT, bool MASK, bool SURF, bool FIELD>
@@ -892,7 +917,7 @@ void destroy_ctx(Ctx *c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (auto e : c->ev) (void)hipEventDestroy(e);
   void *bufs[] = {c->X, c->Y, c->Z, c->PT, c->SX, c->SY, c->SZ, c->surf_d, c->lat_d, c->ctrl_d,
-                  c->blk_d, c->bcount, c->mblk_d, c->partials, c->mom_d, c->obs_d, c->scratch_f};
+                  c->blk_d, c->cnt, c->mblk_d, c->partials, c->mom_d, c->obs_d, c->scratch_f};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -998,6 +1023,7 @@ int create_ctx(const mcl_lattice_desc *lat, int nlat, Ctx *c) {
     L.cy = (float)(d.ny / 2);
     L.cz = (float)(d.nz / 2);
     if (M.elecA != 0.0f || M.elecP != 0.0f) c->field = true;
+    if (d.radial != 0 || c->pot == MCL_POT_PEAR) c->xq = true;
     c->mask |= h.mask;
     c->surf |= h.surf;
     for (int s = 0; s < d.nsurf; s++) {
@@ -1018,7 +1044,7 @@ int create_ctx(const mcl_lattice_desc *lat, int nlat, Ctx *c) {
   }
   c->pool = off;
   // sweep work list: per lattice ceil(nc / kBlock) workgroups of kBlock consecutive colour positions
-  if (off >= (1ll << 31)) return fail(MCL_ERR_ARG, "too many sites in one context (>= 2^31)");
+  if (off >= (1ll << 30)) return fail(MCL_ERR_ARG, "too many sites in one context (>= 2^30)");
   std::vector<int2> blk;
   for (int r = 0; r < nlat; r++) {
     c->ld[r].blk0 = (int)blk.size();
@@ -1047,7 +1073,7 @@ int create_ctx(const mcl_lattice_desc *lat, int nlat, Ctx *c) {
     return st;
   if ((st = dev_alloc(&c->surf_d, surf_h.size())) || (st = dev_alloc(&c->lat_d, nlat)) ||
       (st = dev_alloc(&c->ctrl_d, nlat)) || (st = dev_alloc(&c->blk_d, blk.size())) ||
-      (st = dev_alloc(&c->bcount, 2 * blk.size())) ||
+      (st = dev_alloc(&c->cnt, (size_t)nlat * kCntLattice)) ||
       (st = dev_alloc(&c->mblk_d, mblk.size())) || (st = dev_alloc(&c->partials, mblk.size() * kNQ)) ||
       (st = dev_alloc(&c->mom_d, nlat)) || (st = dev_alloc(&c->obs_d, nlat)) ||
       (st = dev_alloc(&c->scratch_f, (size_t)c->max_nt)))
@@ -1060,9 +1086,14 @@ int create_ctx(const mcl_lattice_desc *lat, int nlat, Ctx *c) {
   MCL_HIP(hipMemcpyAsync(c->mblk_d, mblk.data(), sizeof(int4) * mblk.size(), hipMemcpyHostToDevice, c->stream));
   std::vector<CtrlDev> ctrl(nlat);
   for (int r = 0; r < nlat; r++) {
-    ctrl[r] = CtrlDev{lat[r].T, lat[r].model.elecE, lat[r].ang_var, 0u, 0ull};
+    if (!(lat[r].T >= 0.0f)) return fail(MCL_ERR_ARG, "negative temperature");
+    ctrl[r] = CtrlDev{lat[r].T, lat[r].T > 0.0f ? 1.0f / lat[r].T : 0.0f, lat[r].model.elecE, lat[r].ang_var,
+                      {lat[r].ang_var, lat[r].ang_var}, 0ull};
   }
   MCL_HIP(hipMemcpyAsync(c->ctrl_d, ctrl.data(), sizeof(CtrlDev) * nlat, hipMemcpyHostToDevice, c->stream));
+  std::vector<unsigned int> zeros((size_t)nlat * kCntLattice, 0u);
+  MCL_HIP(hipMemcpyAsync(c->cnt, zeros.data(), sizeof(unsigned int) * zeros.size(), hipMemcpyHostToDevice,
+                         c->stream));
   std::vector<MomDev> mom(nlat, MomDev{0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0});
   MCL_HIP(hipMemcpyAsync(c->mom_d, mom.data(), sizeof(MomDev) * nlat, hipMemcpyHostToDevice, c->stream));
   // per-lattice point types and anchoring vectors in the colour-split layout
@@ -1127,20 +1158,12 @@ int create_ctx(const mcl_lattice_desc *lat, int nlat, Ctx *c) {
   return MCL_OK;
 }
 
-hipEvent_t next_event(Ctx *c) {
-  if (c->ev_used == c->ev.size()) {
-    hipEvent_t e;
-    if (hipEventCreate(&e) != hipSuccess) return nullptr;
-    c->ev.push_back(e);
-  }
-  return c->ev[c->ev_used++];
-}
 
 int run_measure(Ctx *c, int accumulate) {
   const MeasFn mf = pick_meas(*c);
   mf(dim3(c->nmblk), dim3(kBlock), c->stream, *c);
   hipLaunchKernelGGL(k_finalize, dim3(c->nlat), dim3(kBlock), 0, c->stream, c->lat_d, c->partials, c->nmblk,
-                     c->mom_d, c->obs_d, accumulate);
+                     c->xq ? 1 : 0, c->mom_d, c->obs_d, accumulate);
   MCL_HIP(hipGetLastError());
   return MCL_OK;
 }
@@ -1213,25 +1236,30 @@ int mcl_sweeps(void *ctx, int32_t nsweeps, int32_t measure_every) {
   if (nsweeps < 0 || measure_every < 0) return fail(MCL_ERR_ARG, "negative sweep count");
   const SweepFn sf = pick_sweep(*c);
   const dim3 grid(8 * c->chunk), block(kBlock);
+  // Kernel timing: one event pair around each run of consecutive sweep launches (up to timing_stride sweeps, cut
+  // at every measurement), so the markers do not sit between the timed kernels.
+  int run_len = 0;
+  size_t run_start = 0;
   for (int s = 0; s < nsweeps; s++) {
-    const bool timed = c->timing_stride > 0 && (s % c->timing_stride) == 0;
-    if (timed) {
-      hipEvent_t e0 = next_event(c), e1 = next_event(c), e2 = next_event(c);
-      if (!e0 || !e1 || !e2) return fail(MCL_ERR_DEVICE, "hipEventCreate failed");
-      MCL_HIP(hipEventRecord(e0, c->stream));
-      sf(grid, block, c->stream, *c, 0);
-      MCL_HIP(hipEventRecord(e1, c->stream));
-      sf(grid, block, c->stream, *c, 1);
-      MCL_HIP(hipEventRecord(e2, c->stream));
-      c->ev_pairs.push_back({c->ev_used - 3, c->ev_used - 2});
-      c->ev_pairs.push_back({c->ev_used - 2, c->ev_used - 1});
-    } else {
-      sf(grid, block, c->stream, *c, 0);
-      sf(grid, block, c->stream, *c, 1);
+    const unsigned long long sw = c->sweeps_done;
+    if (c->timing_stride > 0 && run_len == 0 && c->ev_used + 2 <= c->ev.size()) {
+      run_start = c->ev_used;
+      c->ev_used += 2;
+      MCL_HIP(hipEventRecord(c->ev[run_start], c->stream));
+      run_len = 1;
+    } else if (run_len > 0) {
+      run_len++;
     }
-    hipLaunchKernelGGL(k_close_sweep, dim3(c->nlat), dim3(kBlock), 0, c->stream, c->lat_d, c->ctrl_d, c->bcount,
-                       c->nblk);
-    if (measure_every > 0 && (s + 1) % measure_every == 0) {
+    sf(grid, block, c->stream, *c, 0, sw, nullptr, nullptr);
+    sf(grid, block, c->stream, *c, 1, sw, nullptr, nullptr);
+    c->sweeps_done = sw + 1;
+    const bool measure_now = measure_every > 0 && (s + 1) % measure_every == 0;
+    if (run_len > 0 && (measure_now || run_len == c->timing_stride || s == nsweeps - 1)) {
+      MCL_HIP(hipEventRecord(c->ev[run_start + 1], c->stream));
+      c->ev_runs.push_back({run_start, 2 * run_len});
+      run_len = 0;
+    }
+    if (measure_now) {
       const int st = run_measure(c, 1);
       if (st) return st;
     }
@@ -1246,9 +1274,9 @@ int mcl_set_kernel_timing(void *ctx, int32_t stride, int32_t max_timed_sweeps) {
   MCL_HIP(hipStreamSynchronize(c->stream));
   c->timing_stride = stride > 0 ? stride : 0;
   c->ev_used = 0;
-  c->ev_pairs.clear();
+  c->ev_runs.clear();
   // create the events up front so that no hipEventCreate falls inside a timed region
-  while (c->ev.size() < 3 * (size_t)std::max(0, max_timed_sweeps)) {
+  while (c->ev.size() < 2 * (size_t)std::max(0, max_timed_sweeps)) {
     hipEvent_t e;
     MCL_HIP(hipEventCreate(&e));
     c->ev.push_back(e);
@@ -1261,12 +1289,14 @@ int mcl_kernel_timing(void *ctx, int64_t *launches, double *total_ms) {
   if (!c || !launches || !total_ms) return fail(MCL_ERR_ARG, "null argument");
   MCL_HIP(hipStreamSynchronize(c->stream));
   double tot = 0.0;
-  for (auto &p : c->ev_pairs) {
+  int64_t n = 0;
+  for (auto &p : c->ev_runs) {
     float ms = 0.0f;
-    MCL_HIP(hipEventElapsedTime(&ms, c->ev[p.first], c->ev[p.second]));
+    MCL_HIP(hipEventElapsedTime(&ms, c->ev[p.first], c->ev[p.first + 1]));
     tot += ms;
+    n += p.second;
   }
-  *launches = (int64_t)c->ev_pairs.size();
+  *launches = n;
   *total_ms = tot;
   return MCL_OK;
 }
@@ -1279,14 +1309,48 @@ int mcl_get_stream(void *ctx, void **stream) {
 }
 
 int mcl_set_temperature(void *ctx, int32_t lat, float T) {
-  if (T < 0.0f) return fail(MCL_ERR_ARG, "negative temperature");
-  return set_ctrl_float(as_ctx(ctx), lat, offsetof(CtrlDev, T), T);
+  if (!(T >= 0.0f)) return fail(MCL_ERR_ARG, "negative temperature");
+  Ctx *c = as_ctx(ctx);
+  int st = set_ctrl_float(c, lat, offsetof(CtrlDev, T), T);
+  if (st) return st;
+  return set_ctrl_float(c, lat, offsetof(CtrlDev, invT), T > 0.0f ? 1.0f / T : 0.0f);
 }
 int mcl_set_field(void *ctx, int32_t lat, float elecE) {
   return set_ctrl_float(as_ctx(ctx), lat, offsetof(CtrlDev, elecE), elecE);
 }
+
 int mcl_set_ang_var(void *ctx, int32_t lat, float ang_var) {
-  return set_ctrl_float(as_ctx(ctx), lat, offsetof(CtrlDev, ang_var), ang_var);
+  Ctx *c = as_ctx(ctx);
+  int st = set_ctrl_float(c, lat, offsetof(CtrlDev, av_set), ang_var);
+  if (st) return st;
+  const int r0 = lat < 0 ? 0 : lat, r1 = lat < 0 ? c->nlat : lat + 1;
+  const unsigned long long sw = c->sweeps_done;
+  for (int r = r0; r < r1; r++)
+    MCL_HIP(hipMemcpyAsync(reinterpret_cast<char *>(c->ctrl_d + r) + offsetof(CtrlDev, av_set_sweep), &sw,
+                           sizeof(sw), hipMemcpyHostToDevice, c->stream));
+  MCL_HIP(hipStreamSynchronize(c->stream));
+  return MCL_OK;
+}
+
+// Accepted rotations of the last completed sweep and the ang_var that the next sweep will use.
+static int sweep_state(Ctx *c, int lat, unsigned *last_acc, float *next_av) {
+  CtrlDev h;
+  std::vector<unsigned> v(kCntSweep, 0u);
+  MCL_HIP(hipMemcpyAsync(&h, c->ctrl_d + lat, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+  const unsigned long long S = c->sweeps_done;
+  unsigned a = 0;
+  if (S > 0) {
+    MCL_HIP(hipMemcpyAsync(v.data(), c->cnt + (size_t)lat * kCntLattice + ((S - 1) % 3) * kCntSweep,
+                           sizeof(unsigned) * kCntSweep, hipMemcpyDeviceToHost, c->stream));
+  }
+  MCL_HIP(hipStreamSynchronize(c->stream));
+  for (int x = 0; x < kCnt; x++) a += v[(size_t)x * kCntStride];
+  *last_acc = a;
+  if (S == h.av_set_sweep)
+    *next_av = h.av_set;
+  else
+    *next_av = ang_var_rule(h.av_store[(S - 1) & 1], a, c->lh[lat].nvalid);
+  return MCL_OK;
 }
 
 int mcl_get_ang_var(void *ctx, int32_t lat, float *ang_var) {
@@ -1294,22 +1358,19 @@ int mcl_get_ang_var(void *ctx, int32_t lat, float *ang_var) {
   int st = check_lat(c, lat);
   if (st) return st;
   if (!ang_var) return fail(MCL_ERR_ARG, "null argument");
-  CtrlDev h;
-  MCL_HIP(hipMemcpyAsync(&h, c->ctrl_d + lat, sizeof(h), hipMemcpyDeviceToHost, c->stream));
-  MCL_HIP(hipStreamSynchronize(c->stream));
-  *ang_var = h.ang_var;
-  return MCL_OK;
+  unsigned a;
+  return sweep_state(c, lat, &a, ang_var);
 }
 
 int mcl_get_sweep_stats(void *ctx, int32_t lat, int64_t *last_accepted, int64_t *sweeps_done) {
   Ctx *c = as_ctx(ctx);
   int st = check_lat(c, lat);
   if (st) return st;
-  CtrlDev h;
-  MCL_HIP(hipMemcpyAsync(&h, c->ctrl_d + lat, sizeof(h), hipMemcpyDeviceToHost, c->stream));
-  MCL_HIP(hipStreamSynchronize(c->stream));
-  if (last_accepted) *last_accepted = (int64_t)h.last_acc;
-  if (sweeps_done) *sweeps_done = (int64_t)h.sweep;
+  unsigned a;
+  float av;
+  if ((st = sweep_state(c, lat, &a, &av))) return st;
+  if (last_accepted) *last_accepted = (int64_t)a;
+  if (sweeps_done) *sweeps_done = (int64_t)c->sweeps_done;
   return MCL_OK;
 }
 

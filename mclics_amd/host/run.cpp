@@ -1,0 +1,261 @@
+// GPU mode loops of the mclics host program (thermalGPU, stepGPU, quenchGPU, electricGPU) and the output
+// writers (po.dat and director_field_*.csv). The loops follow the reference's src/evolve_{thermal,step,quench,
+// electric}.cu: the same schedule, moments, column layout, number formats and file names. The sweeps, the
+// measurements (FP64 on the device) and the local order parameter of the snapshots run in libmclgpu.
+#include "run.h"
+
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <strings.h>
+
+#include <chrono>
+#include <string>
+#include <vector>
+
+namespace {
+
+// Sweep schedule (include/sweep.h:13-34): points counted, not accumulated; snapshot names int(S x).
+int sweep_points(double a, double b, double d) { return (int)floor((b - a) / d + 1e-3) + 1; }
+double sweep_value(double a, double b, double d, int n, int npoints) {
+  if (n == npoints - 1 && fabs((b - a) / d - (npoints - 1)) < 1e-3) return b;
+  return a + n * d;
+}
+int snapshot_scale(double step) {
+  int S = 100;
+  while (S * fabs(step) < 1 - 1e-3 && S < 100000000) S *= 10;
+  return S;
+}
+long snapshot_index(double x, int S) { return (long)(S * (x + 1e-3 / S)); }
+
+[[noreturn]] void gpu_fail(const char *what) {
+  fprintf(stderr, "  [ERROR] GPU (%s): %s\n", what, mcl_last_error());
+  exit(20);
+}
+
+void check(int st, const char *what) {
+  if (st != MCL_OK) gpu_fail(what);
+}
+
+struct Extras {
+  bool pear, sph, cyl;
+  void header(FILE *f) const {
+    if (pear) fprintf(f, " P dtheta divn");
+    if (sph) fprintf(f, pear ? " Sr_sph Pr_sph" : " Sr_sph");
+    if (cyl) fprintf(f, pear ? " Sr_cyl Pr_cyl" : " Sr_cyl");
+    fprintf(f, "\n");
+  }
+  void values(FILE *f, const mcl_moments &m) const {
+    const double n = (double)m.n;
+    if (pear) fprintf(f, " %g %g %g", m.P / n, m.dtheta / n, m.divn / n);
+    if (sph) {
+      fprintf(f, " %g", m.Sr_sph / n);
+      if (pear) fprintf(f, " %g", m.Pr_sph / n);
+    }
+    if (cyl) {
+      fprintf(f, " %g", m.Sr_cyl / n);
+      if (pear) fprintf(f, " %g", m.Pr_cyl / n);
+    }
+    fprintf(f, "\n");
+  }
+};
+
+}  // namespace
+
+GpuRun::GpuRun(Params &p, Lattice &L) : p_(p), L_(L) {
+  mcl_lattice_desc d;
+  memset(&d, 0, sizeof(d));
+  d.nx = L.Nx;
+  d.ny = L.Ny;
+  d.nz = L.Nz;
+  for (int a = 0; a < 3; a++) d.periodic[a] = L.periodic[a];
+  d.ni = L.ni.data();
+  d.pt = L.pt.data();
+  d.ns = L.ns.data();
+  d.nsurf = (int)L.surfaces.size();
+  d.surf = L.surfaces.empty() ? nullptr : L.surfaces.data();
+  mcl_model &m = d.model;
+  m.potential = iequals(p.potential, "pear") ? MCL_POT_PEAR
+                : (iequals(p.potential, "ghrl") || iequals(p.potential, "gruhn-hess")) ? MCL_POT_GHRL
+                                                                                         : MCL_POT_LL;
+  m.A = p.A;
+  m.B1 = p.B1;
+  m.C = p.C;
+  m.ghrl_lambda = p.ghrl_lambda;
+  m.ghrl_mu = p.ghrl_mu;
+  m.ghrl_nu = p.ghrl_nu;
+  m.ghrl_rho = p.ghrl_rho;
+  m.ghrl_sigma = p.ghrl_sigma;
+  m.elec_dir[0] = p.elecX;
+  m.elec_dir[1] = p.elecY;
+  m.elec_dir[2] = p.elecZ;
+  m.elecA = p.elecA;
+  m.elecP = p.elecP;
+  m.elecE = p.elecE;
+  m.flip_move = p.flipMove;
+  d.T = p.Ti;
+  d.ang_var = 0.5f;  // evolve_thermal.cu:43
+  d.seed = (uint64_t)(uint32_t)p.seed;
+  d.radial = (iequals(p.radial, "sphere") || iequals(p.radial, "both") ? MCL_RADIAL_SPHERE : 0) |
+             (iequals(p.radial, "cylinder") || iequals(p.radial, "both") ? MCL_RADIAL_CYLINDER : 0);
+  check(mcl_create(&d, 1, &ctx_), "setup");
+}
+
+GpuRun::~GpuRun() {
+  if (ctx_) mcl_destroy(ctx_);
+}
+
+void GpuRun::sweeps(int n, int measure_every) {
+  if (n <= 0) return;
+  const auto t0 = std::chrono::steady_clock::now();
+  check(mcl_sweeps(ctx_, n, measure_every), "sweep");
+  check(mcl_synchronize(ctx_), "sweep");
+  sweep_seconds_ += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  sweeps_ += n;
+}
+
+// Snapshot and ic.csv format (io.cpp:56-89): header x,y,z,nx,ny,nz,S,pt, x fastest, %g, S = local order
+// parameter, NaN on empty sites.
+int GpuRun::print_n(const char *fname) {
+  const size_t Nt = L_.pt.size();
+  std::vector<float> S(Nt);
+  check(mcl_get_directors(ctx_, 0, L_.ni.data()), "snapshot");
+  check(mcl_local_order(ctx_, 0, S.data()), "snapshot");
+  FILE *out = fopen(fname, "w");
+  if (!out) {
+    perror(fname);
+    return 1;
+  }
+  std::vector<char> buf(1 << 20);
+  setvbuf(out, buf.data(), _IOFBF, buf.size());
+  fprintf(out, "x,y,z,nx,ny,nz,S,pt\n");
+  size_t s = 0;
+  for (int k = 0; k < L_.Nz; k++)
+    for (int j = 0; j < L_.Ny; j++)
+      for (int i = 0; i < L_.Nx; i++, s++)
+        fprintf(out, "%d,%d,%d,%g,%g,%g,%g,%d\n", i, j, k, L_.ni[3 * s], L_.ni[3 * s + 1], L_.ni[3 * s + 2],
+                L_.pt[s] ? S[s] : NAN, L_.pt[s]);
+  printf("  Snapshot -> %s\n", fname);
+  fflush(stdout);
+  fclose(out);
+  return 0;
+}
+
+int GpuRun::run() {
+  const std::string mode = mode_base(p_.evol);
+  Extras ex{iequals(p_.potential, "pear"), iequals(p_.radial, "sphere") || iequals(p_.radial, "both"),
+            iequals(p_.radial, "cylinder") || iequals(p_.radial, "both")};
+  FILE *po = fopen("po.dat", "a");
+  if (!po) {
+    perror("po.dat");
+    return 1;
+  }
+  char fname[1000];
+  mcl_moments m;
+  auto measure_block = [&](void) {
+    check(mcl_reset_moments(ctx_), "moments");
+    sweeps(p_.MCS, 1);
+    check(mcl_get_moments(ctx_, 0, &m), "moments");
+  };
+  auto write_line = [&](const char *first) {
+    const double S1 = m.S / m.n, S2 = m.S2 / m.n, E = m.E / m.n, E2 = m.E2 / m.n;
+    fprintf(po, "%s %g %g %g %g", first, S1, S2 - S1 * S1, E, E2 - E * E);
+    ex.values(po, m);
+    fflush(po);
+  };
+  char first[64];
+  if (iequals(mode, "thermal")) {
+    printf("  Thermal evolution (GPU)\n");
+    print_field("Ti:", p_.Ti);
+    print_field("Tf:", p_.Tf);
+    print_field("dT:", p_.dT);
+    printf("\n");
+    fprintf(po, "T S varS E varE");
+    ex.header(po);
+    fflush(po);
+    const int snapS = snapshot_scale(p_.dT);
+    if (snapS != 100) printf("  Snapshot names: director_field_<int(%d T)>.csv (|dT| < 0.01)\n", snapS);
+    const int nT = sweep_points(p_.Ti, p_.Tf, p_.dT);
+    for (int it = 0; it < nT; it++) {
+      p_.T = (float)sweep_value(p_.Ti, p_.Tf, p_.dT, it, nT);
+      check(mcl_set_temperature(ctx_, 0, p_.T), "temperature");
+      sweeps(p_.MCT, 0);
+      measure_block();
+      snprintf(fname, sizeof(fname), "director_field_%ld.csv", snapshot_index(p_.T, snapS));
+      print_n(fname);
+      snprintf(first, sizeof(first), "%g", p_.T);
+      write_line(first);
+    }
+  } else if (iequals(mode, "step")) {
+    printf("  Step evolution (GPU)\n");
+    print_field("First file:", p_.first_file);
+    print_field("Last file:", p_.first_file + p_.fn);
+    printf("\n");
+    fprintf(po, "ii S varS E varE");
+    ex.header(po);
+    fflush(po);
+    p_.T = p_.Ti;
+    check(mcl_set_temperature(ctx_, 0, p_.T), "temperature");
+    for (int ii = p_.first_file; ii < p_.fn + p_.first_file; ii++) {
+      sweeps(p_.MCT, 0);
+      measure_block();
+      snprintf(fname, sizeof(fname), "director_field_%d.csv", ii);
+      print_n(fname);
+      snprintf(first, sizeof(first), "%d", ii);
+      write_line(first);
+    }
+  } else if (iequals(mode, "quench")) {
+    printf("  Quench evolution (GPU)\n");
+    printf("  %-13s%g (MCT = %d steps)\n", "Ti:", p_.Ti, p_.MCT);
+    printf("  %-13s%g (MCQ = %d steps, then MCS = %d measurement steps)\n", "Tf:", p_.Tf, p_.MCQ, p_.MCS);
+    printf("  %-13s%d\n", "Repetitions:", p_.fn);
+    printf("  %-13s%d\n\n", "First file:", p_.first_file);
+    fprintf(po, "ii S varS E varE");
+    ex.header(po);
+    fflush(po);
+    for (int ii = p_.first_file; ii < p_.fn + p_.first_file; ii++) {
+      p_.T = p_.Ti;
+      check(mcl_set_temperature(ctx_, 0, p_.T), "temperature");
+      sweeps(p_.MCT, 0);
+      p_.T = p_.Tf;
+      check(mcl_set_temperature(ctx_, 0, p_.T), "temperature");
+      sweeps(p_.MCQ, 0);
+      measure_block();
+      snprintf(fname, sizeof(fname), "director_field_%d.csv", ii);
+      print_n(fname);
+      snprintf(first, sizeof(first), "%d", ii);
+      write_line(first);
+    }
+  } else {  // electric
+    printf("  Electric evolution (GPU)\n");
+    print_field("Ei:", p_.elecEi);
+    print_field("Ef:", p_.elecEf);
+    print_field("dE:", p_.elecdE);
+    printf("\n");
+    p_.T = p_.Ti;
+    check(mcl_set_temperature(ctx_, 0, p_.T), "temperature");
+    fprintf(po, "field S varS E varE");
+    ex.header(po);
+    fflush(po);
+    const int snapS = snapshot_scale(p_.elecdE);
+    printf("  Snapshot names: director_field_<int(%d E)>.csv\n", snapS);
+    const int nE = sweep_points(p_.elecEi, p_.elecEf, p_.elecdE);
+    for (int ie = 0; ie < nE; ie++) {
+      p_.elecE = (float)sweep_value(p_.elecEi, p_.elecEf, p_.elecdE, ie, nE);
+      check(mcl_set_field(ctx_, 0, p_.elecE), "field");
+      sweeps(p_.MCT, 0);
+      measure_block();
+      snprintf(fname, sizeof(fname), "director_field_%ld.csv", snapshot_index(p_.elecE, snapS));
+      print_n(fname);
+      snprintf(first, sizeof(first), "%g", p_.elecE);
+      write_line(first);
+    }
+  }
+  fclose(po);
+  const double updates = (double)sweeps_ * (double)L_.nvalid;
+  printf("\n  GPU sweeps: %lld, site updates: %.6g, sweep time: %.3f s (%.3f site-updates/ns)\n\n",
+         (long long)sweeps_, updates, sweep_seconds_,
+         sweep_seconds_ > 0 ? updates / (sweep_seconds_ * 1e9) : 0.0);
+  return 0;
+}

@@ -1,0 +1,313 @@
+"""The mclics program as a drop-in for the reference's GPU modes: the known-answer checks of the reference's
+tests/run_checks.py (run with its --gpu switch, i.e. evol <mode>GPU), re-expressed as pytest.
+
+The input-error checks stop before any device call and run without a GPU; every other check runs the program on
+the GPU. Each check writes an input file into a fresh folder, runs the program there and reads po.dat and the
+snapshots.
+"""
+import math
+import os
+import shutil
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PROGRAM = os.path.join(ROOT, "mclics_amd", "bin", "mclics")
+
+
+class Run:
+    def __init__(self, d, proc):
+        self.dir, self.proc = d, proc
+
+    def po(self):
+        path = os.path.join(self.dir, "po.dat")
+        assert os.path.exists(path), "no po.dat; stdout:\n%s\nstderr:\n%s" % (self.proc.stdout[-2000:],
+                                                                            self.proc.stderr[-2000:])
+        with open(path) as f:
+            lines = [l.split() for l in f if l.strip()]
+        return lines[0], [dict(zip(lines[0], map(float, l))) for l in lines[1:]]
+
+    def ok(self):
+        assert self.proc.returncode == 0, "exit %d\nstdout:\n%s\nstderr:\n%s" % (
+            self.proc.returncode, self.proc.stdout[-3000:], self.proc.stderr[-3000:])
+        return self
+
+    def snapshots(self):
+        return sorted(f for f in os.listdir(self.dir) if f.startswith("director_field_"))
+
+
+@pytest.fixture
+def run(tmp_path):
+    assert os.access(PROGRAM, os.X_OK), "mclics is not built (python -m mclics_amd.build)"
+    count = [0]
+
+    def go(params, files=None, setup=None, raw=False):
+        count[0] += 1
+        d = tmp_path / ("run%03d" % count[0])
+        d.mkdir()
+        lines = []
+        for k, v in (params.items() if isinstance(params, dict) else params):
+            if k.lower() in ("evol", "mode") and not raw and not str(v).lower().endswith("gpu"):
+                v = str(v) + "GPU"
+            lines.append("%s %s" % (k, v))
+        (d / "input.txt").write_text("\n".join(lines) + "\n")
+        for name, text in (files or {}).items():
+            (d / name).write_text(text)
+        if setup:
+            setup(str(d))
+        print("mclics run in %s: %s" % (d, dict(params) if isinstance(params, dict) else params), flush=True)
+        p = subprocess.run([PROGRAM, "input.txt"], cwd=str(d), stdout=subprocess.PIPE, stderr=subprocess.PIPE,
+                           universal_newlines=True, errors="replace", timeout=300)
+        return Run(str(d), p)
+
+    return go
+
+
+def base(**kw):
+    """Small periodic bulk at T = 0 starting from the uniform state along z: one measurement sweep
+    (run_checks.py:97-103)."""
+    p = dict(Nx=8, Ny=8, Nz=8, geometry="bulk", xbound="periodic", ybound="periodic", zbound="periodic",
+             potential="ll", A=1, evol="thermal", Ti=0, Tf=0, dT=-0.1, MCT=0, MCS=1, ic="homogeneous",
+             theta_0=0, phi_0=0, seed=1)
+    p.update(kw)
+    return p
+
+
+def plus(params, *lines):
+    return list(params.items()) + [tuple(l.split(None, 1)) for l in lines]
+
+
+def close(name, value, expected, tol):
+    assert abs(value - expected) <= tol, "%s = %.7g, expected %.7g (tolerance %g)" % (name, value, expected, tol)
+
+
+# ---- input errors: the run stops, reports the problem and leaves the old output alone (run_checks.py:328-380) ----
+
+def expect_error(run, params, text, status=None, raw=False):
+    r = run(params, files={"po.dat": "old results\n"}, raw=raw)
+    assert r.proc.returncode != 0, "the run did not stop"
+    if status is not None:
+        assert r.proc.returncode == status
+    assert text in r.proc.stderr, r.proc.stderr[-500:]
+    with open(os.path.join(r.dir, "po.dat")) as f:
+        assert f.read() == "old results\n", "po.dat of the previous run was changed"
+
+
+def test_error_odd_periodic(run):
+    expect_error(run, base(Nx=5), "requires an even number of sites", 2)
+
+
+def test_error_wrong_dT(run):
+    expect_error(run, base(Ti=0.5, Tf=1.0, dT=-0.1), "does not lead from Ti", 1)
+
+
+def test_error_unknown_parameter(run):
+    expect_error(run, plus(base(), "Tff 0.1"), "Unknown parameter", 3)
+
+
+def test_error_unknown_anchoring(run):
+    expect_error(run, plus(base(Nz=6, geometry="slab"), "anchoring_type 0 strong", "W 0 1", "anchoring_type 1 rp",
+                           "W 1 1", "theta_s 1 90", "phi_s 1 0"), "not implemented", 2)
+
+
+def test_error_missing_anchoring(run):
+    expect_error(run, plus(base(Nz=6, geometry="slab"), "anchoring_type 0 homeotropic", "W 0 1"),
+                 "anchoring_type 1 is not defined", 2)
+
+
+def test_error_ghrl_constants(run):
+    expect_error(run, base(potential="ghrl", k11=5, k22=1, k33=1), "needs k11 < 3 k22 + k33", 1)
+
+
+def test_error_negative_temperature(run):
+    expect_error(run, base(Ti=-0.1, Tf=-0.1), "Ti must be >= 0", 1)
+
+
+def test_error_invalid_value(run):
+    expect_error(run, plus(base(), "MCS many"), "Invalid value for parameter: MCS", 1)
+
+
+def test_error_radial(run):
+    expect_error(run, base(radial="spiral"), "radial must be sphere, cylinder, both or none", 1)
+
+
+def test_error_cpu_mode_rejected(run):
+    """This build has no CPU sweep: a CPU mode name stops with a message and changes nothing."""
+    expect_error(run, base(evol="thermal"), "use thermalGPU", 2, raw=True)
+
+
+def test_error_unknown_mode(run):
+    expect_error(run, base(evol="anneal"), "Evol mode 'annealGPU' not implemented", 2)
+
+
+def test_error_missing_input_file(run, tmp_path):
+    p = subprocess.run([PROGRAM, str(tmp_path / "nope.txt")], stdout=subprocess.PIPE, stderr=subprocess.PIPE,
+                       universal_newlines=True)
+    assert p.returncode == 5
+
+
+def test_error_ic_file_unlisted_site(run):
+    r = run(base(Nx=2, Ny=2, Nz=2, ic="ic_file", ic_file="start.csv"),
+            files={"start.csv": "x,y,z,nx,ny,nz\n0,0,0,0,0,1\n", "po.dat": "old results\n"})
+    assert r.proc.returncode == 3 and "are not listed" in r.proc.stderr
+    assert open(os.path.join(r.dir, "po.dat")).read() == "old results\n"
+
+
+# ---- known answers on the GPU (run_checks.py:113-323) ----
+
+pytestmark_gpu = pytest.mark.gpu
+
+
+@pytestmark_gpu
+def test_ground_state_ll(run):
+    r = run(base()).ok()
+    _, rows = r.po()
+    close("E", rows[0]["E"], -9.0, 1e-4)
+    close("S", rows[0]["S"], 1.0, 1e-5)
+    close("varE", rows[0]["varE"], 0.0, 1e-8)
+
+
+@pytestmark_gpu
+def test_ground_state_ghrl_equal_k(run):
+    _, rows = run(base(potential="ghrl", k11=1, k22=1, k33=1)).ok().po()
+    close("E", rows[0]["E"], -6.0, 1e-4)
+
+
+@pytestmark_gpu
+def test_ground_state_pear(run):
+    _, rows = run(base(potential="pear", A=1.5, B1=0.09, C=0.3)).ok().po()
+    close("E", rows[0]["E"], 6 * (-1.5 - 0.09), 1e-4)
+    close("P", rows[0]["P"], 1.0, 1e-5)
+
+
+@pytestmark_gpu
+def test_ground_state_slab_homeotropic(run):
+    _, rows = run(plus(base(Nz=6, geometry="slab"), "anchoring_type 0 homeotropic", "W 0 1",
+                       "anchoring_type 1 homeotropic", "W 1 1")).ok().po()
+    close("E", rows[0]["E"], (-1.5 * (256 * 6 + 128 * 5) - 128) / 384, 1e-4)
+
+
+@pytestmark_gpu
+def test_ground_state_field(run):
+    _, rows = run(base(elecA=0.5, elecX=0, elecY=0, elecZ=2)).ok().po()
+    close("E", rows[0]["E"], -9.5, 1e-4)
+
+
+def hedgehog_csv(n, sign=1):
+    c = n // 2
+    out = ["x,y,z,nx,ny,nz"]
+    for k in range(n):
+        for j in range(n):
+            for i in range(n):
+                v = (i - c, j - c, k - c)
+                m = math.sqrt(sum(x * x for x in v))
+                u = (0, 0, 1) if m == 0 else tuple(sign * x / m for x in v)
+                out.append("%d,%d,%d,%.6f,%.6f,%.6f" % ((i, j, k) + u))
+    return "\n".join(out) + "\n"
+
+
+@pytestmark_gpu
+def test_hedgehog_radial_order(run):
+    p = plus(base(Nx=21, Ny=21, Nz=21, geometry="sphere", potential="pear", A=1.5, B1=0.09, C=0.3, ic="ic_file",
+                  ic_file="hedgehog.csv"), "anchoring_type 0 homeotropic", "W 0 1.5")
+    header, rows = run(p, files={"hedgehog.csv": hedgehog_csv(21)}).ok().po()
+    for col in ("P", "dtheta", "divn", "Sr_sph", "Pr_sph"):
+        assert col in header
+    close("Sr_sph", rows[0]["Sr_sph"], 1.0, 0.03)
+    close("Pr_sph", rows[0]["Pr_sph"], 1.0, 0.03)
+    close("divn", rows[0]["divn"], 0.275, 0.02)
+
+
+@pytestmark_gpu
+def test_columns_ll_sphere(run):
+    header, _ = run(plus(base(Nx=11, Ny=11, Nz=11, geometry="sphere", ic="random"), "anchoring_type 0 homeotropic",
+                         "W 0 1")).ok().po()
+    assert header == ["T", "S", "varS", "E", "varE", "Sr_sph"]
+
+
+@pytestmark_gpu
+def test_thermal_sweep_points(run):
+    r = run(base(Nx=4, Ny=4, Nz=4, Ti=1.0, Tf=0.5, dT=-0.1, ic="random")).ok()
+    _, rows = r.po()
+    T = [row["T"] for row in rows]
+    assert len(T) == 6 and all(abs(a - b) <= 1e-6 for a, b in zip(T, [1.0, 0.9, 0.8, 0.7, 0.6, 0.5]))
+    assert r.snapshots() == sorted("director_field_%d.csv" % v for v in (100, 90, 80, 70, 60, 50))
+
+
+@pytestmark_gpu
+def test_thermal_fine_step_names(run):
+    r = run(base(Nx=4, Ny=4, Nz=4, Ti=1.24, Tf=1.224, dT=-0.004, ic="random")).ok()
+    _, rows = r.po()
+    assert len(rows) == 5
+    assert r.snapshots() == sorted("director_field_%d.csv" % v for v in (1240, 1236, 1232, 1228, 1224))
+
+
+@pytestmark_gpu
+def test_thermal_step_001_names(run):
+    r = run(base(Nx=4, Ny=4, Nz=4, Ti=1.2, Tf=1.1, dT=-0.01, ic="random")).ok()
+    assert r.snapshots() == sorted("director_field_%d.csv" % v for v in range(110, 121))
+
+
+@pytestmark_gpu
+def test_electric_sweep(run):
+    header, rows = run(base(Nx=4, Ny=4, Nz=4, evol="electric", Ti=0.5, elecA=0.1, elecZ=1, Ei=0, Ef=1.2, dE=0.1,
+                            ic="random")).ok().po()
+    assert header[0] == "field"
+    E = [row["field"] for row in rows]
+    assert len(E) == 13 and abs(E[-1] - 1.2) <= 1e-6
+
+
+@pytestmark_gpu
+def test_step_blocks(run):
+    r = run(base(Nx=4, Ny=4, Nz=4, evol="step", Ti=0.5, fn=3, first_file_number=5, MCT=2, ic="random")).ok()
+    header, rows = r.po()
+    assert header[0] == "ii" and [row["ii"] for row in rows] == [5, 6, 7]
+    assert r.snapshots() == ["director_field_5.csv", "director_field_6.csv", "director_field_7.csv"]
+
+
+@pytestmark_gpu
+def test_quench_relaxes(run):
+    _, rows = run(base(evol="quench", Ti=2.0, Tf=0.5, MCT=50, MCQ=300, MCS=20, fn=1, ic="random")).ok().po()
+    assert rows[0]["S"] > 0.8
+
+
+@pytestmark_gpu
+def test_snapshot_as_ic_file(run):
+    r1 = run(base(Nx=6, Ny=6, Nz=6, Ti=0.3, Tf=0.3, MCT=5, ic="random")).ok()
+    snap = os.path.join(r1.dir, "director_field_30.csv")
+
+    def copy(d):
+        shutil.copy(snap, os.path.join(d, "start.csv"))
+    r2 = run(base(Nx=6, Ny=6, Nz=6, ic="ic_file", ic_file="start.csv"), setup=copy).ok()
+
+    def directors(path):
+        with open(path) as f:
+            next(f)
+            return [tuple(round(float(x), 5) for x in l.split(",")[3:6]) for l in f]
+    assert directors(snap) == directors(os.path.join(r2.dir, "ic.csv"))
+
+
+@pytestmark_gpu
+def test_same_seed_same_run(run):
+    p = base(Nx=6, Ny=6, Nz=6, Ti=0.8, Tf=0.8, MCT=20, MCS=20, ic="random")
+    a, b = run(p).ok(), run(p).ok()
+    p["seed"] = 2
+    c = run(p).ok()
+    po = [open(os.path.join(x.dir, "po.dat")).read() for x in (a, b, c)]
+    assert po[0] == po[1]
+    assert po[0] != po[2]
+
+
+@pytestmark_gpu
+def test_snapshot_format(run):
+    """x,y,z,nx,ny,nz,S,pt with x fastest, NaN for S on empty sites (io.cpp:56-89)."""
+    r = run(plus(base(Nx=7, Ny=7, Nz=7, geometry="sphere", ic="random", MCS=2), "anchoring_type 0 homeotropic",
+                 "W 0 1")).ok()
+    lines = open(os.path.join(r.dir, "ic.csv")).read().splitlines()
+    assert lines[0] == "x,y,z,nx,ny,nz,S,pt" and len(lines) == 1 + 7 ** 3
+    f1 = lines[1].split(",")
+    assert f1[:3] == ["0", "0", "0"] and f1[6] == "nan" and f1[7] == "0"
+    assert lines[2].split(",")[:3] == ["1", "0", "0"]
+    mid = lines[1 + 3 + 7 * (3 + 7 * 3)].split(",")
+    assert mid[:3] == ["3", "3", "3"] and mid[7] == "1" and 0 < float(mid[6]) <= 1.0

@@ -1,0 +1,72 @@
+"""Statistical parity of the GPU sweep with the CPU oracle on the 16^3 Lebwohl-Lasher thermal run.
+
+Case: A = 1, 16x16x16 bulk, periodic x/y/z, random initial directors from the replica's seed, T = 1.2 -> 0.1 with
+dT = -0.02 (56 points), MCT = 500 thermalization and MCS = 200 measurement sweeps per point, 16 replicas
+(seeds 1..16). The oracle's run is the committed fixture tests/golden/c1_oracle.npz (tests/golden/
+make_c1_fixture.py). The GPU runs the 16 replicas as one batched context, starting from the same initial
+directors.
+
+Criterion (S, varS, E, varE at each of the 56 temperatures, 224 comparisons): z = (mean_gpu - mean_cpu) /
+sqrt(s_gpu^2 / 16 + s_cpu^2 / 16) with s the replica-to-replica standard deviation. With 224 comparisons about one
+|z| > 3 is expected by chance, so the test allows at most 4 (p ~ 0.5 % of a false failure) and none beyond 5.
+"""
+import os
+
+import numpy as np
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+FIXTURE = os.path.join(HERE, "golden", "c1_oracle.npz")
+
+pytestmark = pytest.mark.gpu
+
+
+def run_gpu_c1(mcl, orc, seeds):
+    N = 16
+    pt, _ = orc.geometry("bulk", N, N, N)
+    lats = [mcl.Lattice(N, N, N, orc.ic_random(pt, s), periodic=(1, 1, 1), model=mcl.make_model("ll", A=1.0),
+                        T=1.2, ang_var=0.5, seed=s) for s in seeds]
+    f32 = lambda x: float(np.float32(x))
+    Ti, Tf, dT = f32(1.2), f32(0.1), f32(-0.02)
+    nT = orc.sweep_points(Ti, Tf, dT)
+    rows = np.zeros((len(seeds), nT, 5))
+    with mcl.Engine(lats) as eng:
+        for it in range(nT):
+            T = float(np.float32(orc.sweep_value(Ti, Tf, dT, it, nT)))
+            eng.set_temperature(T)
+            eng.sweeps(500)
+            eng.reset_moments()
+            eng.sweeps(200, 1)
+            for r in range(len(seeds)):
+                m = eng.moments(r)
+                S1, S2, E1, E2 = m["S"] / m["n"], m["S2"] / m["n"], m["E"] / m["n"], m["E2"] / m["n"]
+                rows[r, it] = (T, S1, S2 - S1 * S1, E1, E2 - E1 * E1)
+    return rows
+
+
+def z_scores(gpu, cpu):
+    """gpu, cpu: (replicas, points) arrays -> z per point."""
+    n_g, n_c = gpu.shape[0], cpu.shape[0]
+    d = gpu.mean(0) - cpu.mean(0)
+    s = np.sqrt(gpu.var(0, ddof=1) / n_g + cpu.var(0, ddof=1) / n_c)
+    s = np.where(s > 0, s, 1e-300)
+    return d / s
+
+
+def test_c1_thermal_parity_16_replicas(mcl, orc):
+    ref = np.load(FIXTURE)["rows"]  # (16, 56, 6): T S varS E varE Sr
+    seeds = list(range(1, ref.shape[0] + 1))
+    gpu = run_gpu_c1(mcl, orc, seeds)
+    assert np.allclose(gpu[0, :, 0], ref[0, :, 0], atol=1e-6)
+    names = ["S", "varS", "E", "varE"]
+    zs = np.stack([z_scores(gpu[:, :, c], ref[:, :, c]) for c in (1, 2, 3, 4)])  # (4, 56)
+    worst = np.unravel_index(np.argmax(np.abs(zs)), zs.shape)
+    report = "; ".join("%s max|z| %.2f at T=%.2f" % (names[c], np.abs(zs[c]).max(),
+                                                     ref[0, int(np.argmax(np.abs(zs[c]))), 0]) for c in range(4))
+    print(report)
+    n3 = int((np.abs(zs) > 3).sum())
+    assert n3 <= 4, "%d of 224 comparisons beyond 3 sigma: %s" % (n3, report)
+    assert np.abs(zs).max() < 5, "worst %s at T=%.2f: z=%.2f (%s)" % (names[worst[0]], ref[0, worst[1], 0],
+                                                                      zs[worst], report)
+    # the sampled physics: ordered at low T, disordered at high T
+    assert gpu[:, -1, 1].mean() > 0.95 and gpu[:, 0, 1].mean() < 0.15
