@@ -73,6 +73,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2000)
     ap.add_argument("--timing-stride", type=int, default=10, help="time the kernels of every n-th sweep")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-bytes", type=int, default=256 << 20,
+                    help="size of the STREAM-style copy that checks the HBM peak (0: skip)")
     a = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if a.gpus != 1 or world != 1:
@@ -104,6 +106,10 @@ def main():
         mom = eng.moments()
         acc, done = eng.sweep_stats()
         av = eng.ang_var()
+    copy_gbs = None
+    if a.copy_bytes > 0:
+        copy_gbs, copy_ms = capi.copy_bandwidth(a.copy_bytes, 20)
+        log("bench: device copy of %d MiB: %.0f GB/s (read + write)" % (a.copy_bytes >> 20, copy_gbs))
     elapsed = t1 - t0
     value = a.steps * nt / (elapsed * 1e9)
     avg_launch_s = kms / launches / 1e3
@@ -129,7 +135,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(),
                      "kernel": "k_sweep (one colour pass)", "bytes_per_launch": bytes_per_launch,
-                     "avg_launch_us": round(avg_launch_s * 1e6, 3), "timed_launches": launches},
+                     "avg_launch_us": round(avg_launch_s * 1e6, 3), "timed_launches": launches,
+                     "copy_measured_gbs": None if copy_gbs is None else round(copy_gbs, 1)},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

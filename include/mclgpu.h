@@ -167,6 +167,10 @@ int mcl_philox(const uint32_t *ctr, const uint32_t *key, int32_t n, uint32_t *ou
  * (synchronizes). stride = 0 switches timing off. */
 int mcl_set_kernel_timing(void *ctx, int32_t stride, int32_t max_timed_sweeps);
 int mcl_kernel_timing(void *ctx, int64_t *launches, double *total_ms);
+/* STREAM-style copy of nbytes (rounded down to 16 B) between two fresh device buffers, reps times on the null
+ * stream; returns the bandwidth counting bytes read plus written, and the time per copy. It confirms the HBM peak
+ * and calibrates the FETCH_SIZE / WRITE_SIZE counters with a known byte count. */
+int mcl_copy_bandwidth(int64_t nbytes, int32_t reps, double *gbs, double *ms_per_copy);
 /* Device stream of the context (a hipStream_t), for callers that record their own events. */
 int mcl_get_stream(void *ctx, void **stream);
 

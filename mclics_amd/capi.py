@@ -89,6 +89,8 @@ ABI = {
     "mcl_kernel_timing": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64),
                                          ctypes.POINTER(ctypes.c_double)]),
     "mcl_get_stream": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)]),
+    "mcl_copy_bandwidth": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int32, ctypes.POINTER(ctypes.c_double),
+                                          ctypes.POINTER(ctypes.c_double)]),
 }
 
 _lib = None
@@ -282,3 +284,10 @@ def philox(ctr, key):
     P = ctypes.POINTER(ctypes.c_uint32)
     _check(lib.mcl_philox(c.ctypes.data_as(P), k.ctypes.data_as(P), len(c), out.ctypes.data_as(P)))
     return out
+
+
+def copy_bandwidth(nbytes, reps=20):
+    """STREAM-style device copy: (GB/s counting read + write bytes, ms per copy)."""
+    g, ms = ctypes.c_double(), ctypes.c_double()
+    _check(load().mcl_copy_bandwidth(int(nbytes), int(reps), ctypes.byref(g), ctypes.byref(ms)))
+    return g.value, ms.value

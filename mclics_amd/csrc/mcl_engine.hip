@@ -744,6 +744,13 @@ __global__ void k_site_energies(const LatDev *__restrict__ lats, const CtrlDev *
   e_trial[g] = site_energy<POT, SURF, FIELD>(L, s, tv, surf, elecE);
 }
 
+// Plain copy (16 bytes per lane per iteration, grid-stride): the STREAM-style kernel used to confirm the HBM peak
+// and to calibrate the FETCH_SIZE / WRITE_SIZE counters against a known number of bytes.
+__global__ __launch_bounds__(kBlock) void k_copy(const float4 *__restrict__ src, float4 *__restrict__ dst, size_t n) {
+  const size_t stride = (size_t)gridDim.x * kBlock;
+  for (size_t g = (size_t)blockIdx.x * kBlock + threadIdx.x; g < n; g += stride) dst[g] = src[g];
+}
+
 __global__ void k_philox(const uint32_t *__restrict__ ctr, const uint32_t *__restrict__ key, int n,
                          uint32_t *__restrict__ out) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1555,6 +1562,44 @@ int mcl_philox(const uint32_t *ctr, const uint32_t *key, int32_t n, uint32_t *ou
   (void)hipFree(d_key);
   (void)hipFree(d_out);
   if (e != hipSuccess) return fail(MCL_ERR_DEVICE, std::string("philox: ") + hipGetErrorString(e));
+  return MCL_OK;
+}
+
+int mcl_copy_bandwidth(int64_t nbytes, int32_t reps, double *gbs, double *ms_per_copy) {
+  if (nbytes < 16 || reps < 1 || !gbs) return fail(MCL_ERR_ARG, "copy: bad arguments");
+  const size_t n = (size_t)nbytes / 16;
+  float4 *src = nullptr, *dst = nullptr;
+  int st;
+  if ((st = dev_alloc(&src, n)) || (st = dev_alloc(&dst, n))) {
+    (void)hipFree(src);
+    (void)hipFree(dst);
+    return st;
+  }
+  hipEvent_t a = nullptr, b = nullptr;
+  hipError_t e = hipMemset(src, 0, n * sizeof(float4));
+  if (e == hipSuccess) e = hipEventCreate(&a);
+  if (e == hipSuccess) e = hipEventCreate(&b);
+  const dim3 grid(256 * 16);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_copy, grid, dim3(kBlock), 0, 0, src, dst, n);  // warm-up
+    e = hipEventRecord(a, 0);
+  }
+  for (int r = 0; e == hipSuccess && r < reps; r++) {
+    hipLaunchKernelGGL(k_copy, grid, dim3(kBlock), 0, 0, src, dst, n);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipEventRecord(b, 0);
+  if (e == hipSuccess) e = hipEventSynchronize(b);
+  float ms = 0.0f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, a, b);
+  if (a) (void)hipEventDestroy(a);
+  if (b) (void)hipEventDestroy(b);
+  (void)hipFree(src);
+  (void)hipFree(dst);
+  if (e != hipSuccess) return fail(MCL_ERR_DEVICE, std::string("copy: ") + hipGetErrorString(e));
+  const double per = (double)ms / reps;
+  *gbs = 2.0 * (double)(n * sizeof(float4)) / (per * 1e6);  // bytes read + written
+  if (ms_per_copy) *ms_per_copy = per;
   return MCL_OK;
 }
 
