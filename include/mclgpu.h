@@ -89,7 +89,8 @@ typedef struct mcl_lattice_desc {
   mcl_model model;
   float T;                   /* temperature */
   float ang_var;             /* initial maximum rotation (0.5 in the reference, evolve_thermal.cu:43) */
-  uint64_t seed;             /* RNG key: independent streams for different (seed, lattice index) */
+  uint64_t seed;             /* RNG key: one independent stream per seed (the same seed gives the same stream at any
+                                position of any context, so replicas need distinct seeds) */
   int32_t radial;            /* enum mcl_radial */
 } mcl_lattice_desc;
 
@@ -171,6 +172,30 @@ int mcl_kernel_timing(void *ctx, int64_t *launches, double *total_ms);
  * stream; returns the bandwidth counting bytes read plus written, and the time per copy. It confirms the HBM peak
  * and calibrates the FETCH_SIZE / WRITE_SIZE counters with a known byte count. */
 int mcl_copy_bandwidth(int64_t nbytes, int32_t reps, double *gbs, double *ms_per_copy);
+/* ---- launch shape, graphs and the autotuner (no reference counterpart: the reference launches 8 kernels of a
+ * fixed shape per sweep, evolve.cu:334-339 and 416-422) ---- */
+
+/* Result of the autotuner: threads per workgroup, tile order (1: consecutive tiles on the same XCD), measured
+ * time per sweep of the context's lattices, and whether it came from the cache. */
+typedef struct mcl_tune_result {
+  int32_t threads;
+  int32_t xcd_order;
+  double us_per_sweep;
+  int32_t from_cache;
+} mcl_tune_result;
+
+/* Times every candidate launch shape (64..1024 threads per workgroup x tile order) over trial_sweeps sweeps of
+ * the context's own lattices and applies the fastest. The simulation state (directors, ang_var, counters, sweep
+ * number) is saved before and restored after the trials. Results are cached per shape signature in the process
+ * and, when the environment variable MCL_TUNE_CACHE names a file, in that file. */
+int mcl_autotune(void *ctx, int32_t trial_sweeps, mcl_tune_result *out);
+/* Sets the launch shape by hand (threads a multiple of 64, at most 1024). */
+int mcl_set_launch(void *ctx, int32_t threads, int32_t xcd_order);
+/* block_sweeps > 0: mcl_sweeps replays captured HIP graphs of about block_sweeps sweeps (a multiple of
+ * measure_every) instead of launching each kernel; the rest of a call runs eagerly. 0 switches graphs off.
+ * Results are identical to the eager launches. */
+int mcl_set_graph(void *ctx, int32_t block_sweeps);
+
 /* Device stream of the context (a hipStream_t), for callers that record their own events. */
 int mcl_get_stream(void *ctx, void **stream);
 

@@ -55,6 +55,11 @@ class Moments(ctypes.Structure):
                                            "dtheta", "divn")]
 
 
+class TuneResult(ctypes.Structure):
+    _fields_ = [("threads", ctypes.c_int32), ("xcd_order", ctypes.c_int32), ("us_per_sweep", ctypes.c_double),
+                ("from_cache", ctypes.c_int32)]
+
+
 class Observables(ctypes.Structure):
     _fields_ = [("nvalid", ctypes.c_int64), ("E", ctypes.c_double), ("Q", ctypes.c_double * 6),
                 ("S", ctypes.c_double), ("director", ctypes.c_double * 3)] + \
@@ -89,6 +94,9 @@ ABI = {
     "mcl_kernel_timing": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64),
                                          ctypes.POINTER(ctypes.c_double)]),
     "mcl_get_stream": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)]),
+    "mcl_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(TuneResult)]),
+    "mcl_set_launch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]),
+    "mcl_set_graph": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "mcl_copy_bandwidth": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int32, ctypes.POINTER(ctypes.c_double),
                                           ctypes.POINTER(ctypes.c_double)]),
 }
@@ -262,6 +270,18 @@ class Engine:
         _check(self.lib.mcl_site_energies(self.h, lat, s.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), _fptr(t),
                                           len(s), _fptr(ec), _fptr(et)))
         return ec, et
+
+    def autotune(self, trial_sweeps=20):
+        r = TuneResult()
+        _check(self.lib.mcl_autotune(self.h, int(trial_sweeps), ctypes.byref(r)))
+        return {"threads": r.threads, "xcd_order": r.xcd_order, "us_per_sweep": r.us_per_sweep,
+                "from_cache": bool(r.from_cache)}
+
+    def set_launch(self, threads, xcd_order=1):
+        _check(self.lib.mcl_set_launch(self.h, int(threads), int(xcd_order)))
+
+    def set_graph(self, block_sweeps):
+        _check(self.lib.mcl_set_graph(self.h, int(block_sweeps)))
 
     def synchronize(self):
         _check(self.lib.mcl_synchronize(self.h))

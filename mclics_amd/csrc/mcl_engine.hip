@@ -70,8 +70,6 @@ struct LatDev {
   int nvalid;
   uint32_t off;   // pooled offset of the colour-0 block; colour 1 follows at off + nc (pool < 2^31 sites)
   uint32_t key0, key1;
-  int nblk;       // sweep workgroups per colour
-  int blk0;       // first sweep workgroup of this lattice in the work list
   int surf0;      // first entry of this lattice in the pooled surface table
   int mp0, mp1;   // measurement partials [mp0, mp1)
   float A, B1, C, gl, gm, gn, gr, gs;
@@ -313,22 +311,27 @@ __host__ __device__ __forceinline__ float ang_var_rule(float av, unsigned acc, i
 // workgroup L; consecutive logical workgroups are placed on the same XCD (workgroups are dispatched to the 8 XCDs
 // round robin), so the rows a workgroup shares with its neighbours in z and y mostly hit the same L2.
 template <int POT, bool MASK, bool SURF, bool FIELD, bool FLIP>
-__global__ __launch_bounds__(kBlock) void k_sweep(const LatDev *__restrict__ lats, CtrlDev *__restrict__ ctrl,
-                                                  unsigned int *__restrict__ cnt, const int2 *__restrict__ blk,
-                                                  int nblk, int chunk, int colour, unsigned long long sweep,
+// Launch shape (chosen by the autotuner): any multiple of 64 up to 1024 threads per workgroup; with xcd != 0
+// workgroup b runs logical tile (b & 7) * chunk + (b >> 3), so consecutive tiles share an XCD and its L2.
+// The sweep number is *sweep_base + k: graphs replay fixed arguments while the base advances on the device.
+__global__ __launch_bounds__(1024) void k_sweep(const LatDev *__restrict__ lats, CtrlDev *__restrict__ ctrl,
+                                                unsigned int *__restrict__ cnt, const int2 *__restrict__ blk,
+                                                int nblk, int chunk, int xcd, int colour,
+                                                const unsigned long long *__restrict__ sweep_base, unsigned k,
                                                   float *__restrict__ X, float *__restrict__ Y,
                                                   float *__restrict__ Z, const uint8_t *__restrict__ PT,
                                                   const float *__restrict__ SX, const float *__restrict__ SY,
                                                   const float *__restrict__ SZ, const SurfDev *__restrict__ surf) {
   const int b = blockIdx.x;
-  const int Lb = (b & 7) * chunk + (b >> 3);
+  const int Lb = xcd ? (b & 7) * chunk + (b >> 3) : b;
   if (Lb >= nblk) return;
   const int2 bd = blk[Lb];
   const int r = bd.x;
   const LatDev &L = lats[r];
+  const unsigned long long sweep = *sweep_base + k;
   const float T = ctrl[r].T, invT = ctrl[r].invT, elecE = ctrl[r].elecE;
   const int lane = threadIdx.x & 63;
-  const int lblk = bd.y / kBlock;  // workgroup index within the lattice
+  const int lblk = bd.y / (int)blockDim.x;  // workgroup index within the lattice
   unsigned int *const cnt_r = cnt + (size_t)r * kCntLattice;
   __shared__ float av_sh;
   float av;
@@ -407,14 +410,14 @@ __global__ __launch_bounds__(kBlock) void k_sweep(const LatDev *__restrict__ lat
 
   // accepted rotations: wave ballot, then one atomic per workgroup into one of kCnt counters (striping keeps the
   // device-scope atomics of thousands of workgroups from serializing on one address)
-  __shared__ unsigned int wsum[kWaves];
+  __shared__ unsigned int wsum[16];
   const unsigned long long ball = __ballot(accepted ? 1 : 0);
   if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = (unsigned)__popcll(ball);
   __syncthreads();
   if (threadIdx.x == 0) {
     unsigned tot = 0;
-#pragma unroll
-    for (int w = 0; w < kWaves; w++) tot += wsum[w];
+    const int nw = (int)(blockDim.x >> 6);
+    for (int w = 0; w < nw; w++) tot += wsum[w];
     if (tot) atomicAdd(&cnt_r[(sweep % 3) * kCntSweep + (lblk & (kCnt - 1)) * kCntStride], tot);
   }
 }
@@ -665,6 +668,10 @@ __global__ __launch_bounds__(kBlock) void k_finalize(const LatDev *__restrict__ 
   }
 }
 
+__global__ void k_advance(unsigned long long *sweep_base, unsigned n) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *sweep_base += n;
+}
+
 __global__ void k_zero_moments(MomDev *mom, int n) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r < n) mom[r] = MomDev{0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
@@ -804,8 +811,12 @@ struct Ctx {
   CtrlDev *ctrl_d = nullptr;
   int2 *blk_d = nullptr;
   unsigned int *cnt = nullptr;  // striped acceptance counters [lattice][sweep % 3][kCnt]
-  unsigned long long sweeps_done = 0;
+  unsigned long long sweeps_done = 0;            // host copy of the device sweep base
+  unsigned long long *sweep_base = nullptr;      // device sweep base (advanced at the end of every mcl_sweeps)
   int nblk = 0, chunk = 0;
+  int bs = 256, xcd = 1;  // sweep launch shape: threads per workgroup, XCD-contiguous tile order
+  int graph_block = 0;    // > 0: sweeps replayed from captured graphs of about this many sweeps
+  std::vector<std::pair<std::pair<int, int>, hipGraphExec_t>> graphs;  // ((sweeps, measure_every), graph)
   int4 *mblk_d = nullptr;
   int nmblk = 0;
   double *partials = nullptr;
@@ -820,16 +831,13 @@ struct Ctx {
   std::vector<std::pair<size_t, int>> ev_runs;  // (first event of the pair, kernel launches in the run)
 };
 
-using SweepFn = void (*)(dim3, dim3, hipStream_t, const Ctx &, int, unsigned long long, hipEvent_t, hipEvent_t);
+using SweepFn = void (*)(dim3, dim3, hipStream_t, const Ctx &, int, unsigned);
 using MeasFn = void (*)(dim3, dim3, hipStream_t, const Ctx &);
 
-// Launched through hipExtLaunchKernelGGL so that timed launches get start/stop events from the dispatch itself
-// (no extra queue packets between kernels).
 template <int POT, bool MASK, bool SURF, bool FIELD, bool FLIP>
-void launch_sweep(dim3 g, dim3 b, hipStream_t st, const Ctx &c, int colour, unsigned long long sweep, hipEvent_t e0,
-                  hipEvent_t e1) {
-  hipExtLaunchKernelGGL((k_sweep<POT, MASK, SURF, FIELD, FLIP>), g, b, 0, st, e0, e1, 0, c.lat_d, c.ctrl_d, c.cnt,
-                        c.blk_d, c.nblk, c.chunk, colour, sweep, c.X, c.Y, c.Z, c.PT, c.SX, c.SY, c.SZ, c.surf_d);
+void launch_sweep(dim3 g, dim3 b, hipStream_t st, const Ctx &c, int colour, unsigned k) {
+  hipLaunchKernelGGL((k_sweep<POT, MASK, SURF, FIELD, FLIP>), g, b, 0, st, c.lat_d, c.ctrl_d, c.cnt, c.blk_d, c.nblk,
+                     c.chunk, c.xcd, colour, c.sweep_base, k, c.X, c.Y, c.Z, c.PT, c.SX, c.SY, c.SZ, c.surf_d);
 }
 template <int POT, bool MASK, bool SURF, bool FIELD>
 void launch_measure(dim3 g, dim3 b, hipStream_t st, const Ctx &c) {
@@ -923,8 +931,9 @@ void destroy_ctx(Ctx *c) {
   if (!c) return;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (auto e : c->ev) (void)hipEventDestroy(e);
+  for (auto &g : c->graphs) (void)hipGraphExecDestroy(g.second);
   void *bufs[] = {c->X, c->Y, c->Z, c->PT, c->SX, c->SY, c->SZ, c->surf_d, c->lat_d, c->ctrl_d,
-                  c->blk_d, c->cnt, c->mblk_d, c->partials, c->mom_d, c->obs_d, c->scratch_f};
+                  c->blk_d, c->cnt, c->sweep_base, c->mblk_d, c->partials, c->mom_d, c->obs_d, c->scratch_f};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -937,6 +946,29 @@ int dev_alloc(T **p, size_t n) {
   hipError_t e = hipMalloc(&q, sizeof(T) * (n ? n : 1));
   if (e != hipSuccess) return fail(MCL_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
   *p = static_cast<T *>(q);
+  return MCL_OK;
+}
+
+// Sweep work list for workgroups of bs threads: per lattice ceil(nc / bs) workgroups of bs consecutive colour
+// positions. Drops the captured graphs, which hold the old launch shape.
+int build_worklist(Ctx *c, int bs, int xcd) {
+  std::vector<int2> blk;
+  for (int r = 0; r < c->nlat; r++)
+    for (int t0 = 0; t0 < c->ld[r].nc; t0 += bs) blk.push_back(make_int2(r, t0));
+  if (c->blk_d) {
+    MCL_HIP(hipStreamSynchronize(c->stream));
+    MCL_HIP(hipFree(c->blk_d));
+    c->blk_d = nullptr;
+  }
+  int st;
+  if ((st = dev_alloc(&c->blk_d, blk.size()))) return st;
+  MCL_HIP(hipMemcpy(c->blk_d, blk.data(), sizeof(int2) * blk.size(), hipMemcpyHostToDevice));
+  c->nblk = (int)blk.size();
+  c->chunk = (c->nblk + 7) / 8;
+  c->bs = bs;
+  c->xcd = xcd;
+  for (auto &g : c->graphs) (void)hipGraphExecDestroy(g.second);
+  c->graphs.clear();
   return MCL_OK;
 }
 
@@ -1009,8 +1041,7 @@ int create_ctx(const mcl_lattice_desc *lat, int nlat, Ctx *c) {
     L.nvalid = h.nvalid;
     L.off = (uint32_t)h.off;
     L.key0 = (uint32_t)d.seed;
-    L.key1 = (uint32_t)(d.seed >> 32) ^ (0x9E3779B9u * (uint32_t)(r + 1));
-    L.nblk = (h.nc + kBlock - 1) / kBlock;
+    L.key1 = (uint32_t)(d.seed >> 32) ^ 0x5851F42Du;  // the stream depends on the seed only, not on the position
     L.surf0 = (int)surf_h.size();
     const mcl_model &M = d.model;
     L.A = M.A;
@@ -1052,13 +1083,6 @@ int create_ctx(const mcl_lattice_desc *lat, int nlat, Ctx *c) {
   c->pool = off;
   // sweep work list: per lattice ceil(nc / kBlock) workgroups of kBlock consecutive colour positions
   if (off >= (1ll << 30)) return fail(MCL_ERR_ARG, "too many sites in one context (>= 2^30)");
-  std::vector<int2> blk;
-  for (int r = 0; r < nlat; r++) {
-    c->ld[r].blk0 = (int)blk.size();
-    for (int b = 0; b < c->ld[r].nblk; b++) blk.push_back(make_int2(r, b * kBlock));
-  }
-  c->nblk = (int)blk.size();
-  c->chunk = (c->nblk + 7) / 8;
   // measurement work list: per lattice and colour, workgroups of kBlock * kMeasPerThread positions
   std::vector<int4> mblk;
   const int span = kBlock * kMeasPerThread;
@@ -1079,7 +1103,7 @@ int create_ctx(const mcl_lattice_desc *lat, int nlat, Ctx *c) {
                   (st = dev_alloc(&c->SZ, c->pool))))
     return st;
   if ((st = dev_alloc(&c->surf_d, surf_h.size())) || (st = dev_alloc(&c->lat_d, nlat)) ||
-      (st = dev_alloc(&c->ctrl_d, nlat)) || (st = dev_alloc(&c->blk_d, blk.size())) ||
+      (st = dev_alloc(&c->ctrl_d, nlat)) || (st = dev_alloc(&c->sweep_base, 1)) ||
       (st = dev_alloc(&c->cnt, (size_t)nlat * kCntLattice)) ||
       (st = dev_alloc(&c->mblk_d, mblk.size())) || (st = dev_alloc(&c->partials, mblk.size() * kNQ)) ||
       (st = dev_alloc(&c->mom_d, nlat)) || (st = dev_alloc(&c->obs_d, nlat)) ||
@@ -1089,7 +1113,9 @@ int create_ctx(const mcl_lattice_desc *lat, int nlat, Ctx *c) {
     MCL_HIP(hipMemcpyAsync(c->surf_d, surf_h.data(), sizeof(SurfDev) * surf_h.size(), hipMemcpyHostToDevice,
                            c->stream));
   MCL_HIP(hipMemcpyAsync(c->lat_d, c->ld.data(), sizeof(LatDev) * nlat, hipMemcpyHostToDevice, c->stream));
-  MCL_HIP(hipMemcpyAsync(c->blk_d, blk.data(), sizeof(int2) * blk.size(), hipMemcpyHostToDevice, c->stream));
+  const unsigned long long zero_base = 0ull;
+  MCL_HIP(hipMemcpyAsync(c->sweep_base, &zero_base, sizeof(zero_base), hipMemcpyHostToDevice, c->stream));
+  if ((st = build_worklist(c, 256, 1))) return st;
   MCL_HIP(hipMemcpyAsync(c->mblk_d, mblk.data(), sizeof(int4) * mblk.size(), hipMemcpyHostToDevice, c->stream));
   std::vector<CtrlDev> ctrl(nlat);
   for (int r = 0; r < nlat; r++) {
@@ -1237,39 +1263,97 @@ int mcl_create(const mcl_lattice_desc *lat, int32_t nlat, void **ctx_out) {
 
 void mcl_destroy(void *ctx) { destroy_ctx(as_ctx(ctx)); }
 
+// Queues sweeps [k0, k0 + n) relative to the device sweep base (no timing, no advance).
+static int enqueue_sweeps(Ctx *c, SweepFn sf, int k0, int n, int measure_every) {
+  const dim3 grid(c->xcd ? 8 * c->chunk : c->nblk), block(c->bs);
+  for (int s = 0; s < n; s++) {
+    sf(grid, block, c->stream, *c, 0, (unsigned)(k0 + s));
+    sf(grid, block, c->stream, *c, 1, (unsigned)(k0 + s));
+    if (measure_every > 0 && (s + 1) % measure_every == 0) {
+      const int st = run_measure(c, 1);
+      if (st) return st;
+    }
+  }
+  return MCL_OK;
+}
+
+// Graph of `len` sweeps (with the measurements of measure_every) followed by the advance of the sweep base.
+static int get_graph(Ctx *c, SweepFn sf, int len, int measure_every, hipGraphExec_t *out) {
+  for (auto &g : c->graphs)
+    if (g.first.first == len && g.first.second == measure_every) {
+      *out = g.second;
+      return MCL_OK;
+    }
+  hipGraph_t graph = nullptr;
+  MCL_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  int st = enqueue_sweeps(c, sf, 0, len, measure_every);
+  hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, c->stream, c->sweep_base, (unsigned)len);
+  const hipError_t e = hipStreamEndCapture(c->stream, &graph);
+  if (st) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return st;
+  }
+  if (e != hipSuccess) return fail(MCL_ERR_DEVICE, std::string("graph capture: ") + hipGetErrorString(e));
+  hipGraphExec_t exec = nullptr;
+  const hipError_t e2 = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (e2 != hipSuccess) return fail(MCL_ERR_DEVICE, std::string("graph instantiate: ") + hipGetErrorString(e2));
+  c->graphs.push_back({{len, measure_every}, exec});
+  *out = exec;
+  return MCL_OK;
+}
+
 int mcl_sweeps(void *ctx, int32_t nsweeps, int32_t measure_every) {
   Ctx *c = as_ctx(ctx);
   if (!c) return fail(MCL_ERR_ARG, "null context");
   if (nsweeps < 0 || measure_every < 0) return fail(MCL_ERR_ARG, "negative sweep count");
+  if (nsweeps == 0) return MCL_OK;
   const SweepFn sf = pick_sweep(*c);
-  const dim3 grid(8 * c->chunk), block(kBlock);
-  // Kernel timing: one event pair around each run of consecutive sweep launches (up to timing_stride sweeps, cut
-  // at every measurement), so the markers do not sit between the timed kernels.
-  int run_len = 0;
-  size_t run_start = 0;
-  for (int s = 0; s < nsweeps; s++) {
-    const unsigned long long sw = c->sweeps_done;
-    if (c->timing_stride > 0 && run_len == 0 && c->ev_used + 2 <= c->ev.size()) {
-      run_start = c->ev_used;
-      c->ev_used += 2;
-      MCL_HIP(hipEventRecord(c->ev[run_start], c->stream));
-      run_len = 1;
-    } else if (run_len > 0) {
-      run_len++;
-    }
-    sf(grid, block, c->stream, *c, 0, sw, nullptr, nullptr);
-    sf(grid, block, c->stream, *c, 1, sw, nullptr, nullptr);
-    c->sweeps_done = sw + 1;
-    const bool measure_now = measure_every > 0 && (s + 1) % measure_every == 0;
-    if (run_len > 0 && (measure_now || run_len == c->timing_stride || s == nsweeps - 1)) {
-      MCL_HIP(hipEventRecord(c->ev[run_start + 1], c->stream));
-      c->ev_runs.push_back({run_start, 2 * run_len});
-      run_len = 0;
-    }
-    if (measure_now) {
-      const int st = run_measure(c, 1);
+  const dim3 grid(c->xcd ? 8 * c->chunk : c->nblk), block(c->bs);
+  int done = 0;
+  if (c->graph_block > 0 && c->timing_stride == 0) {
+    // whole graphs of len sweeps (a multiple of measure_every, so the measurements fall where the eager path
+    // puts them), then the rest eagerly
+    int len = c->graph_block;
+    if (measure_every > 0) len = measure_every * std::max(1, (c->graph_block + measure_every / 2) / measure_every);
+    if (nsweeps >= len) {
+      hipGraphExec_t g;
+      int st = get_graph(c, sf, len, measure_every, &g);
       if (st) return st;
+      for (; done + len <= nsweeps; done += len) MCL_HIP(hipGraphLaunch(g, c->stream));
+      c->sweeps_done += (unsigned long long)done;
     }
+  }
+  const int rest = nsweeps - done;
+  if (rest > 0) {
+    // Kernel timing: one event pair around each run of consecutive sweep launches (up to timing_stride sweeps,
+    // cut at every measurement), so the markers do not sit between the timed kernels.
+    int run_len = 0;
+    size_t run_start = 0;
+    for (int s = 0; s < rest; s++) {
+      if (c->timing_stride > 0 && run_len == 0 && c->ev_used + 2 <= c->ev.size()) {
+        run_start = c->ev_used;
+        c->ev_used += 2;
+        MCL_HIP(hipEventRecord(c->ev[run_start], c->stream));
+        run_len = 1;
+      } else if (run_len > 0) {
+        run_len++;
+      }
+      sf(grid, block, c->stream, *c, 0, (unsigned)s);
+      sf(grid, block, c->stream, *c, 1, (unsigned)s);
+      const bool measure_now = measure_every > 0 && (s + 1) % measure_every == 0;
+      if (run_len > 0 && (measure_now || run_len == c->timing_stride || s == rest - 1)) {
+        MCL_HIP(hipEventRecord(c->ev[run_start + 1], c->stream));
+        c->ev_runs.push_back({run_start, 2 * run_len});
+        run_len = 0;
+      }
+      if (measure_now) {
+        const int st = run_measure(c, 1);
+        if (st) return st;
+      }
+    }
+    hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, c->stream, c->sweep_base, (unsigned)rest);
+    c->sweeps_done += (unsigned long long)rest;
   }
   MCL_HIP(hipGetLastError());
   return MCL_OK;
@@ -1600,6 +1684,178 @@ int mcl_copy_bandwidth(int64_t nbytes, int32_t reps, double *gbs, double *ms_per
   const double per = (double)ms / reps;
   *gbs = 2.0 * (double)(n * sizeof(float4)) / (per * 1e6);  // bytes read + written
   if (ms_per_copy) *ms_per_copy = per;
+  return MCL_OK;
+}
+
+int mcl_set_graph(void *ctx, int32_t block_sweeps) {
+  Ctx *c = as_ctx(ctx);
+  if (!c) return fail(MCL_ERR_ARG, "null context");
+  if (block_sweeps < 0) return fail(MCL_ERR_ARG, "negative graph block");
+  c->graph_block = block_sweeps;
+  return MCL_OK;
+}
+
+int mcl_set_launch(void *ctx, int32_t threads, int32_t xcd_order) {
+  Ctx *c = as_ctx(ctx);
+  if (!c) return fail(MCL_ERR_ARG, "null context");
+  if (threads < 64 || threads > 1024 || threads % 64) return fail(MCL_ERR_ARG, "threads must be 64..1024, x64");
+  return build_worklist(c, threads, xcd_order ? 1 : 0);
+}
+
+}  // extern "C"
+
+namespace {
+
+std::string tune_key(const Ctx *c) {
+  int maxnc = 0;
+  for (const auto &h : c->lh) maxnc = std::max(maxnc, h.nc);
+  int lp = 0, lm = 0;
+  while ((1ll << lp) < c->pool) lp++;
+  while ((1 << lm) < maxnc) lm++;
+  char buf[160];
+  snprintf(buf, sizeof(buf), "sweep:pot%d:mask%d:surf%d:field%d:flip%d:nlat%d:pool2^%d:maxnc2^%d", c->pot, c->mask,
+           c->surf, c->field, c->flip, c->nlat, lp, lm);
+  return buf;
+}
+
+std::vector<std::pair<std::string, mcl_tune_result>> g_tune_cache;
+
+bool tune_lookup(const std::string &key, mcl_tune_result *out) {
+  for (auto &e : g_tune_cache)
+    if (e.first == key) {
+      *out = e.second;
+      return true;
+    }
+  const char *path = getenv("MCL_TUNE_CACHE");
+  if (!path) return false;
+  FILE *f = fopen(path, "r");
+  if (!f) return false;
+  char k[200];
+  int th, xo;
+  double us;
+  bool found = false;
+  while (fscanf(f, "%199s %d %d %lf", k, &th, &xo, &us) == 4)
+    if (key == k) {
+      out->threads = th;
+      out->xcd_order = xo;
+      out->us_per_sweep = us;
+      found = true;
+    }
+  fclose(f);
+  if (found) g_tune_cache.push_back({key, *out});
+  return found;
+}
+
+void tune_store(const std::string &key, const mcl_tune_result &r) {
+  g_tune_cache.push_back({key, r});
+  const char *path = getenv("MCL_TUNE_CACHE");
+  if (!path) return;
+  FILE *f = fopen(path, "a");
+  if (!f) return;
+  fprintf(f, "%s %d %d %.4f\n", key.c_str(), r.threads, r.xcd_order, r.us_per_sweep);
+  fclose(f);
+}
+
+}  // namespace
+
+extern "C" {
+
+// Autotuner of the sweep launch shape: times every (threads per workgroup, tile order) candidate on the context's
+// own lattices and keeps the fastest. The lattices, control blocks, counters and sweep count are saved before the
+// trials and restored after them, so tuning leaves the simulation state untouched.
+int mcl_autotune(void *ctx, int32_t trial_sweeps, mcl_tune_result *out) {
+  Ctx *c = as_ctx(ctx);
+  if (!c || !out) return fail(MCL_ERR_ARG, "null argument");
+  if (trial_sweeps < 1) return fail(MCL_ERR_ARG, "trial_sweeps must be >= 1");
+  const std::string key = tune_key(c);
+  mcl_tune_result best{};
+  if (tune_lookup(key, &best)) {
+    best.from_cache = 1;
+    int st = build_worklist(c, best.threads, best.xcd_order);
+    if (st) return st;
+    *out = best;
+    return MCL_OK;
+  }
+  MCL_HIP(hipStreamSynchronize(c->stream));
+  // save the state the trials change
+  float *bx = nullptr, *by = nullptr, *bz = nullptr;
+  CtrlDev *bctrl = nullptr;
+  unsigned *bcnt = nullptr;
+  int st;
+  const size_t pool = (size_t)c->pool, ncnt = (size_t)c->nlat * kCntLattice;
+  auto release = [&]() {
+    (void)hipFree(bx);
+    (void)hipFree(by);
+    (void)hipFree(bz);
+    (void)hipFree(bctrl);
+    (void)hipFree(bcnt);
+  };
+  if ((st = dev_alloc(&bx, pool)) || (st = dev_alloc(&by, pool)) || (st = dev_alloc(&bz, pool)) ||
+      (st = dev_alloc(&bctrl, c->nlat)) || (st = dev_alloc(&bcnt, ncnt))) {
+    release();
+    return st;
+  }
+  hipError_t e = hipMemcpyAsync(bx, c->X, sizeof(float) * pool, hipMemcpyDeviceToDevice, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(by, c->Y, sizeof(float) * pool, hipMemcpyDeviceToDevice, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(bz, c->Z, sizeof(float) * pool, hipMemcpyDeviceToDevice, c->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(bctrl, c->ctrl_d, sizeof(CtrlDev) * c->nlat, hipMemcpyDeviceToDevice, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(bcnt, c->cnt, sizeof(unsigned) * ncnt, hipMemcpyDeviceToDevice, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) {
+    release();
+    return fail(MCL_ERR_DEVICE, std::string("autotune save: ") + hipGetErrorString(e));
+  }
+  const unsigned long long saved_sweeps = c->sweeps_done;
+  const int saved_graph = c->graph_block, saved_timing = c->timing_stride;
+  c->graph_block = 0;
+  c->timing_stride = 0;
+  hipEvent_t a = nullptr, b = nullptr;
+  (void)hipEventCreate(&a);
+  (void)hipEventCreate(&b);
+  const int threads[] = {64, 128, 256, 512, 1024};
+  best.us_per_sweep = 1e30;
+  for (int th : threads) {
+    for (int xo = 0; xo < 2 && st == MCL_OK && e == hipSuccess; xo++) {
+      if ((st = build_worklist(c, th, xo))) break;
+      if ((st = mcl_sweeps(c, 2, 0))) break;  // warm-up
+      if ((e = hipEventRecord(a, c->stream)) != hipSuccess) break;
+      if ((st = mcl_sweeps(c, trial_sweeps, 0))) break;
+      if ((e = hipEventRecord(b, c->stream)) != hipSuccess) break;
+      if ((e = hipEventSynchronize(b)) != hipSuccess) break;
+      float ms = 0.0f;
+      if ((e = hipEventElapsedTime(&ms, a, b)) != hipSuccess) break;
+      const double us = 1e3 * ms / trial_sweeps;
+      if (us < best.us_per_sweep) {
+        best.us_per_sweep = us;
+        best.threads = th;
+        best.xcd_order = xo;
+      }
+    }
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  // restore the state
+  const unsigned long long base = saved_sweeps;
+  hipError_t r = hipMemcpyAsync(c->X, bx, sizeof(float) * pool, hipMemcpyDeviceToDevice, c->stream);
+  if (r == hipSuccess) r = hipMemcpyAsync(c->Y, by, sizeof(float) * pool, hipMemcpyDeviceToDevice, c->stream);
+  if (r == hipSuccess) r = hipMemcpyAsync(c->Z, bz, sizeof(float) * pool, hipMemcpyDeviceToDevice, c->stream);
+  if (r == hipSuccess)
+    r = hipMemcpyAsync(c->ctrl_d, bctrl, sizeof(CtrlDev) * c->nlat, hipMemcpyDeviceToDevice, c->stream);
+  if (r == hipSuccess) r = hipMemcpyAsync(c->cnt, bcnt, sizeof(unsigned) * ncnt, hipMemcpyDeviceToDevice, c->stream);
+  if (r == hipSuccess) r = hipMemcpyAsync(c->sweep_base, &base, sizeof(base), hipMemcpyHostToDevice, c->stream);
+  if (r == hipSuccess) r = hipStreamSynchronize(c->stream);
+  release();
+  c->sweeps_done = saved_sweeps;
+  c->graph_block = saved_graph;
+  c->timing_stride = saved_timing;
+  if (st) return st;
+  if (e != hipSuccess) return fail(MCL_ERR_DEVICE, std::string("autotune: ") + hipGetErrorString(e));
+  if (r != hipSuccess) return fail(MCL_ERR_DEVICE, std::string("autotune restore: ") + hipGetErrorString(r));
+  best.from_cache = 0;
+  if ((st = build_worklist(c, best.threads, best.xcd_order))) return st;
+  tune_store(key, best);
+  *out = best;
   return MCL_OK;
 }
 

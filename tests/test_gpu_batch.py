@@ -1,0 +1,101 @@
+"""Batching, launch shapes, graph replay and the autotuner (needs an MI355X).
+
+The RNG counter is (site, sweep) and the acceptance count is an exact integer sum, so a lattice's trajectory does
+not depend on the launch shape, on graph replay or on the other lattices of its context: all of these must give
+bit-identical directors and moments.
+"""
+import numpy as np
+import pytest
+
+from mclics_amd import batch
+
+pytestmark = pytest.mark.gpu
+
+SPECS = [dict(L=24, geometry="bulk", T=1.05, seed=11), dict(L=10, geometry="bulk", T=0.7, seed=12),
+         dict(L=14, geometry="sphere", T=0.9, seed=13, W=1.0), dict(L=16, geometry="bulk", T=1.25, seed=14)]
+
+
+def state(eng, n):
+    return [eng.directors(r) for r in range(n)], [eng.moments(r) for r in range(n)], \
+           [eng.ang_var(r) for r in range(n)], [eng.sweep_stats(r) for r in range(n)]
+
+
+def same(a, b):
+    dirs_a, mom_a, av_a, st_a = a
+    dirs_b, mom_b, av_b, st_b = b
+    return all(np.array_equal(x, y) for x, y in zip(dirs_a, dirs_b)) and mom_a == mom_b and av_a == av_b and \
+        st_a == st_b
+
+
+def run(mcl, shape=None, graph=0, calls=((30, 5),)):
+    with mcl.Engine([batch.build_lattice(s) for s in SPECS]) as eng:
+        if shape:
+            eng.set_launch(*shape)
+        if graph:
+            eng.set_graph(graph)
+        out = []
+        for n, every in calls:
+            eng.sweeps(n, every)
+            out.append(state(eng, len(SPECS)))
+        return out
+
+
+@pytest.mark.parametrize("shape", [(64, 0), (128, 1), (512, 0), (1024, 1)])
+def test_launch_shape_does_not_change_the_trajectory(mcl, shape):
+    ref = run(mcl, (256, 1))
+    got = run(mcl, shape)
+    assert same(ref[-1], got[-1])
+
+
+def test_graph_replay_matches_eager(mcl):
+    """Several graph replays with eager calls and host synchronizations between them; every call's state must
+    equal the eager run's."""
+    calls = ((40, 4), (7, 0), (24, 4), (33, 3), (64, 0))
+    eager = run(mcl, calls=calls)
+    graph = run(mcl, graph=8, calls=calls)
+    for k, (a, b) in enumerate(zip(eager, graph)):
+        assert same(a, b), "call %d differs" % k
+
+
+def test_batched_lattice_equals_lattice_alone(mcl):
+    with mcl.Engine([batch.build_lattice(s) for s in SPECS]) as eng:
+        eng.sweeps(25, 5)
+        together = state(eng, len(SPECS))
+    for r, s in enumerate(SPECS):
+        with mcl.Engine(batch.build_lattice(s)) as eng:
+            eng.sweeps(25, 5)
+            alone = state(eng, 1)
+        assert np.array_equal(alone[0][0], together[0][r])
+        # the measurement kernels of the two contexts are different template instances (the batch has a sphere:
+        # masked, radial), which may contract the FP32 site energy differently by an ulp
+        for k, v in alone[1][0].items():
+            assert together[1][r][k] == pytest.approx(v, rel=1e-12, abs=1e-12)
+
+
+def test_autotune_keeps_the_state(mcl):
+    with mcl.Engine([batch.build_lattice(s) for s in SPECS]) as eng, \
+            mcl.Engine([batch.build_lattice(s) for s in SPECS]) as twin:
+        eng.sweeps(12, 3)
+        twin.sweeps(12, 3)
+        before = state(eng, len(SPECS))
+        res = eng.autotune(5)
+        assert res["threads"] in (64, 128, 256, 512, 1024) and res["us_per_sweep"] > 0
+        assert same(before, state(eng, len(SPECS)))
+        again = eng.autotune(5)
+        assert again["from_cache"] and again["threads"] == res["threads"]
+        eng.sweeps(20, 4)
+        twin.sweeps(20, 4)
+        assert same(state(eng, len(SPECS)), state(twin, len(SPECS)))
+
+
+def test_small_batch_physics(mcl):
+    """A small version of the batched configuration: ordered lattices at low T, disordered at high T, FP64 moments
+    and chi from every lattice."""
+    specs = batch.c4_specs(24, seed=3, lmin=10, lmax=16, tmin=0.5, tmax=1.8)
+    rows, info = batch.run_batch(specs, 300, 300, every=5, autotune=True, graph=16)
+    T, S = rows[:, 0], rows[:, 1]
+    bulk = np.array([s["geometry"] == "bulk" for s in specs])
+    assert np.all(S[bulk & (T < 0.8)] > 0.7)
+    assert np.all(S[bulk & (T > 1.5)] < 0.4)
+    assert np.all(rows[:, 2] >= 0) and np.all(rows[:, 4] >= 0) and np.all(np.isfinite(rows))
+    assert info["site_updates_per_ns"] > 0
