@@ -67,9 +67,9 @@ def test_batched_lattice_equals_lattice_alone(mcl):
             alone = state(eng, 1)
         assert np.array_equal(alone[0][0], together[0][r])
         # the measurement kernels of the two contexts are different template instances (the batch has a sphere:
-        # masked, radial), which may contract the FP32 site energy differently by an ulp
+        # masked, radial), which may contract the FP32 site energy differently by an ulp (1e-7 relative per site)
         for k, v in alone[1][0].items():
-            assert together[1][r][k] == pytest.approx(v, rel=1e-12, abs=1e-12)
+            assert together[1][r][k] == pytest.approx(v, rel=1e-6, abs=1e-9)
 
 
 def test_autotune_keeps_the_state(mcl):
