@@ -199,6 +199,30 @@ int mcl_set_graph(void *ctx, int32_t block_sweeps);
 /* Device stream of the context (a hipStream_t), for callers that record their own events. */
 int mcl_get_stream(void *ctx, void **stream);
 
+/* ---- the learned surrogate (no reference counterpart) ----
+ * An MLP 5 -> 64 SiLU -> 64 SiLU -> 3 in FP32, trained and evaluated by HIP kernels. Inputs are (T, A, W, field,
+ * log N) and outputs (S, E, chi) of the plain Metropolis method, both normalized by the caller. Parameter layout:
+ * W1[64][5], b1[64], W2[64][64], b2[64], W3[3][64], b3[3]. It only chooses where the plain method samples and
+ * warm-starts directors; it produces no reported number. */
+int mcl_mlp_create(int32_t n_in, int32_t n_hidden, int32_t n_out, uint64_t seed, void **mlp);
+void mcl_mlp_destroy(void *mlp);
+int mcl_mlp_num_params(void);
+int mcl_mlp_get_params(void *mlp, float *params, int32_t n);
+/* Sets the parameters and resets the Adam moments. */
+int mcl_mlp_set_params(void *mlp, const float *params, int32_t n);
+/* y[n][3] = f(x[n][5]) (host arrays). */
+int mcl_mlp_forward(void *mlp, const float *x, int32_t n, float *y);
+/* Gradient of L = (1/n) sum_s sum_o w_o (f(x_s)_o - t_so)^2 with respect to the parameters, and L. w_out may be
+ * NULL (all ones). */
+int mcl_mlp_gradient(void *mlp, const float *x, const float *t, const float *w_out, int32_t n, float *grad,
+                     float *loss);
+/* Uploads the training set and runs `epochs` epochs of mini-batch Adam (beta 0.9 / 0.999, eps 1e-8) with
+ * learning rate lr, shuffling with a generator keyed by seed; returns the mean loss of the last epoch. */
+int mcl_mlp_set_data(void *mlp, const float *x, const float *t, int32_t n);
+int mcl_mlp_train(void *mlp, int32_t epochs, int32_t batch, float lr, const float *w_out, uint64_t seed,
+                  float *last_loss);
+const char *mcl_mlp_last_error(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -30,9 +30,9 @@ def _run(cmd, cwd=ROOT):
 
 def build_lib():
     os.makedirs(os.path.dirname(LIB), exist_ok=True)
-    src = os.path.join(HERE, "csrc", "mcl_engine.hip")
+    srcs = [os.path.join(HERE, "csrc", f) for f in ("mcl_engine.hip", "mcl_mlp.hip")]
     tmp = LIB + ".tmp"
-    _run([HIPCC] + HIP_FLAGS + ["-shared", "-o", tmp, src])
+    _run([HIPCC] + HIP_FLAGS + ["-shared", "-o", tmp] + srcs)
     os.replace(tmp, LIB)
 
 

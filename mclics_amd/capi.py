@@ -97,6 +97,22 @@ ABI = {
     "mcl_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(TuneResult)]),
     "mcl_set_launch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]),
     "mcl_set_graph": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
+    "mcl_mlp_create": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_uint64,
+                                      ctypes.POINTER(ctypes.c_void_p)]),
+    "mcl_mlp_destroy": (None, [ctypes.c_void_p]),
+    "mcl_mlp_num_params": (ctypes.c_int, []),
+    "mcl_mlp_get_params": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float), ctypes.c_int32]),
+    "mcl_mlp_set_params": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float), ctypes.c_int32]),
+    "mcl_mlp_forward": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float), ctypes.c_int32,
+                                       ctypes.POINTER(ctypes.c_float)]),
+    "mcl_mlp_gradient": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float),
+                                        ctypes.POINTER(ctypes.c_float), ctypes.c_int32, ctypes.POINTER(ctypes.c_float),
+                                        ctypes.POINTER(ctypes.c_float)]),
+    "mcl_mlp_set_data": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float),
+                                        ctypes.c_int32]),
+    "mcl_mlp_train": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_float,
+                                     ctypes.POINTER(ctypes.c_float), ctypes.c_uint64, ctypes.POINTER(ctypes.c_float)]),
+    "mcl_mlp_last_error": (ctypes.c_char_p, []),
     "mcl_copy_bandwidth": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int32, ctypes.POINTER(ctypes.c_double),
                                           ctypes.POINTER(ctypes.c_double)]),
 }
@@ -311,3 +327,62 @@ def copy_bandwidth(nbytes, reps=20):
     g, ms = ctypes.c_double(), ctypes.c_double()
     _check(load().mcl_copy_bandwidth(int(nbytes), int(reps), ctypes.byref(g), ctypes.byref(ms)))
     return g.value, ms.value
+
+
+def _mlp_check(code):
+    if code != MCL_OK:
+        raise MclError(code, load().mcl_mlp_last_error().decode(errors="replace"))
+
+
+class Mlp:
+    """The surrogate network (5 -> 64 SiLU -> 64 SiLU -> 3) with HIP forward, gradient and Adam kernels."""
+
+    def __init__(self, seed=1):
+        self.lib = load()
+        h = ctypes.c_void_p()
+        _mlp_check(self.lib.mcl_mlp_create(5, 64, 3, int(seed), ctypes.byref(h)))
+        self.h = h
+        self.nparams = self.lib.mcl_mlp_num_params()
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.mcl_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def params(self):
+        p = np.empty(self.nparams, dtype=np.float32)
+        _mlp_check(self.lib.mcl_mlp_get_params(self.h, _fptr(p), self.nparams))
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        _mlp_check(self.lib.mcl_mlp_set_params(self.h, _fptr(p), self.nparams))
+
+    def forward(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, 5)
+        y = np.empty((len(x), 3), dtype=np.float32)
+        _mlp_check(self.lib.mcl_mlp_forward(self.h, _fptr(x), len(x), _fptr(y)))
+        return y
+
+    def gradient(self, x, t, w_out=None):
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, 5)
+        t = np.ascontiguousarray(t, dtype=np.float32).reshape(-1, 3)
+        w = None if w_out is None else np.ascontiguousarray(w_out, dtype=np.float32)
+        g = np.empty(self.nparams, dtype=np.float32)
+        loss = ctypes.c_float()
+        _mlp_check(self.lib.mcl_mlp_gradient(self.h, _fptr(x), _fptr(t), None if w is None else _fptr(w), len(x),
+                                             _fptr(g), ctypes.byref(loss)))
+        return g, loss.value
+
+    def train(self, x, t, epochs, batch=512, lr=1e-3, w_out=None, seed=1):
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, 5)
+        t = np.ascontiguousarray(t, dtype=np.float32).reshape(-1, 3)
+        w = None if w_out is None else np.ascontiguousarray(w_out, dtype=np.float32)
+        _mlp_check(self.lib.mcl_mlp_set_data(self.h, _fptr(x), _fptr(t), len(x)))
+        loss = ctypes.c_float()
+        _mlp_check(self.lib.mcl_mlp_train(self.h, int(epochs), int(batch), float(lr), None if w is None else _fptr(w),
+                                          int(seed), ctypes.byref(loss)))
+        return loss.value

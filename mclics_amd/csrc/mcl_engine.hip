@@ -528,10 +528,6 @@ __global__ __launch_bounds__(kBlock) void k_measure(const LatDev *__restrict__ l
         if (nbc) div += dd / nbc;
       }
       q[20] += div;
-    } else if constexpr (XQ) {
-      q[13] += x;
-      q[14] += y;
-      q[15] += z;
     }
   }
   __shared__ double red[kWaves][kNQ];
