@@ -12,6 +12,7 @@ periodic bulk and half spheres (radius N/2, homeotropic W = 1.0, free boundaries
 [0.9, 1.3] and per-lattice seeds.
 """
 import argparse
+import functools
 import json
 import math
 import sys
@@ -22,9 +23,11 @@ import numpy as np
 from . import capi
 
 
+@functools.lru_cache(maxsize=64)
 def sphere_geometry(n):
     """Point types and outward normals of the sphere geometry (geometry_sphere.cpp:32-58): bulk for
-    r < Nz/2 - 1, surface 0 for r < Nz/2 + 1, empty outside, centre (N/2, N/2, N/2) with integer division."""
+    r < Nz/2 - 1, surface 0 for r < Nz/2 + 1, empty outside, centre (N/2, N/2, N/2) with integer division.
+    Cached per size; the arrays are read-only."""
     h = np.float32(n // 2)
     k, j, i = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
     rx, ry, rz = (i - h).astype(np.float32), (j - h).astype(np.float32), (k - h).astype(np.float32)
@@ -36,7 +39,10 @@ def sphere_geometry(n):
         ns[..., 0] = np.where(surf, rx / r, 0)
         ns[..., 1] = np.where(surf, ry / r, 0)
         ns[..., 2] = np.where(surf, rz / r, 0)
-    return pt.reshape(-1), ns.reshape(-1)
+    pt, ns = pt.reshape(-1), ns.reshape(-1)
+    pt.flags.writeable = False
+    ns.flags.writeable = False
+    return pt, ns
 
 
 def random_directors(pt, seed):
