@@ -707,46 +707,75 @@ int orc_snapshot_scale(double step) {
 }
 long orc_snapshot_index(double x, int S) { return (long)(S * (x + 1e-3 / S)); }
 
-/* thermalEvolveN::run, evolve_thermal.cpp:144-212 (one RNG per thread seeded from seed*104729 + thread). */
-int orc_thermal_run(orc_params *p, float *ni, const int *pt, const float *ns, double Ti, double Tf, double dT,
-                    int MCT, int MCS, uint64_t seed, int nthreads, double *rows, int max_rows) {
+/* One point of a mode loop: MCT sweeps, then MCS sweeps with one measurement each (evolve_thermal.cpp:160-200,
+ * evolve_electric.cpp:68-90). row = (x, S, varS, E, varE, Sr_sph or NaN). */
+static void orc_point(orc_params *p, float *ni, const int *pt, const float *ns, int MCT, int MCS, float *ang_var,
+                      orc_rng *rngs, int nthreads, int radial, double x, double *row) {
   const int Nt = p->Nx * p->Ny * p->Nz;
-  orc_rng *rngs = (orc_rng *)calloc((size_t)nthreads, sizeof(orc_rng));
-  for (int t = 0; t < nthreads; t++) orc_rng_seed(&rngs[t], seed * 104729ULL + (uint64_t)t, 0x5EED);
-  float ang_var = 0.5f;
-  const int nT = orc_sweep_points(Ti, Tf, dT);
-  int sph = 1;
-  for (int it = 0; it < nT && it < max_rows; it++) {
-    p->T = (float)orc_sweep_value(Ti, Tf, dT, it, nT);
-    for (int step = 0; step < MCT; step++) orc_sweep(p, ni, pt, ns, &ang_var, rngs, nthreads);
-    double S1 = 0, S2 = 0, E = 0, E2 = 0, Sr = 0;
-    for (int step = 0; step < MCS; step++) {
-      orc_sweep(p, ni, pt, ns, &ang_var, rngs, nthreads);
-      double e = orc_energy_mean(p, ni, pt, ns);
-      E += e;
-      E2 += e * e;
-      double Q[9], v[3];
-      orc_q_tensor(Nt, ni, pt, Q);
-      double s = orc_eig_sym3(Q, v, NULL);
-      S1 += s;
-      S2 += s * s;
+  for (int step = 0; step < MCT; step++) orc_sweep(p, ni, pt, ns, ang_var, rngs, nthreads);
+  double S1 = 0, S2 = 0, E = 0, E2 = 0, Sr = 0;
+  for (int step = 0; step < MCS; step++) {
+    orc_sweep(p, ni, pt, ns, ang_var, rngs, nthreads);
+    double e = orc_energy_mean(p, ni, pt, ns);
+    E += e;
+    E2 += e * e;
+    double Q[9], v[3];
+    orc_q_tensor(Nt, ni, pt, Q);
+    double s = orc_eig_sym3(Q, v, NULL);
+    S1 += s;
+    S2 += s * s;
+    if (radial) {
       double sr, pr;
       orc_radial_order(p, ni, pt, 0, &sr, &pr);
       Sr += sr;
     }
-    E /= MCS;
-    E2 /= MCS;
-    S1 /= MCS;
-    S2 /= MCS;
-    rows[6 * it + 0] = p->T;
-    rows[6 * it + 1] = S1;
-    rows[6 * it + 2] = S2 - S1 * S1;
-    rows[6 * it + 3] = E;
-    rows[6 * it + 4] = E2 - E * E;
-    rows[6 * it + 5] = sph ? Sr / MCS : NAN;
+  }
+  E /= MCS;
+  E2 /= MCS;
+  S1 /= MCS;
+  S2 /= MCS;
+  row[0] = x;
+  row[1] = S1;
+  row[2] = S2 - S1 * S1;
+  row[3] = E;
+  row[4] = E2 - E * E;
+  row[5] = radial ? Sr / MCS : NAN;
+}
+
+static orc_rng *orc_thread_rngs(uint64_t seed, int nthreads) {
+  orc_rng *rngs = (orc_rng *)calloc((size_t)nthreads, sizeof(orc_rng));
+  for (int t = 0; t < nthreads; t++) orc_rng_seed(&rngs[t], seed * 104729ULL + (uint64_t)t, 0x5EED);
+  return rngs;
+}
+
+/* thermalEvolveN::run, evolve_thermal.cpp:144-212 (one RNG per thread seeded from seed*104729 + thread). */
+int orc_thermal_run(orc_params *p, float *ni, const int *pt, const float *ns, double Ti, double Tf, double dT,
+                    int MCT, int MCS, uint64_t seed, int nthreads, double *rows, int max_rows) {
+  orc_rng *rngs = orc_thread_rngs(seed, nthreads);
+  float ang_var = 0.5f;
+  const int nT = orc_sweep_points(Ti, Tf, dT);
+  for (int it = 0; it < nT && it < max_rows; it++) {
+    p->T = (float)orc_sweep_value(Ti, Tf, dT, it, nT);
+    orc_point(p, ni, pt, ns, MCT, MCS, &ang_var, rngs, nthreads, 1, p->T, rows + 6 * it);
   }
   free(rngs);
   return nT < max_rows ? nT : max_rows;
+}
+
+/* electricEvolveN::run, evolve_electric.cpp:31-103: field magnitude swept Ei, Ei + dE, ..., Ef at the fixed
+ * temperature p->T, ang_var carried from point to point (starts at 0.5). rows: nE x 6 (field, S, varS, E, varE,
+ * NaN); returns nE. */
+int orc_electric_run(orc_params *p, float *ni, const int *pt, const float *ns, double Ei, double Ef, double dE,
+                     int MCT, int MCS, uint64_t seed, int nthreads, double *rows, int max_rows) {
+  orc_rng *rngs = orc_thread_rngs(seed, nthreads);
+  float ang_var = 0.5f;
+  const int nE = orc_sweep_points(Ei, Ef, dE);
+  for (int ie = 0; ie < nE && ie < max_rows; ie++) {
+    p->elecE = (float)orc_sweep_value(Ei, Ef, dE, ie, nE);
+    orc_point(p, ni, pt, ns, MCT, MCS, &ang_var, rngs, nthreads, 0, p->elecE, rows + 6 * ie);
+  }
+  free(rngs);
+  return nE < max_rows ? nE : max_rows;
 }
 
 double orc_time_sweeps(const orc_params *p, float *ni, const int *pt, const float *ns, int nsweeps, uint64_t seed,

@@ -117,6 +117,10 @@ double orc_splay_divergence(const orc_params *p, const float *ni, const int *pt)
  * measurement each. rows: nT x 6 doubles (T, S, varS, E, varE, Sr_sph); returns nT. ang_var starts at 0.5. */
 int orc_thermal_run(orc_params *p, float *ni, const int *pt, const float *ns, double Ti, double Tf, double dT,
                     int MCT, int MCS, uint64_t seed, int nthreads, double *rows, int max_rows);
+/* Electric mode (evolve_electric.cpp:31-103): field magnitude swept Ei -> Ef at the fixed T = p->T. rows: nE x 6
+ * doubles (field, S, varS, E, varE, NaN); returns nE. ang_var starts at 0.5. */
+int orc_electric_run(orc_params *p, float *ni, const int *pt, const float *ns, double Ei, double Ef, double dE,
+                     int MCT, int MCS, uint64_t seed, int nthreads, double *rows, int max_rows);
 /* Timed CPU baseline: nsweeps sweeps; returns wall seconds. */
 double orc_time_sweeps(const orc_params *p, float *ni, const int *pt, const float *ns, int nsweeps, uint64_t seed,
                        int nthreads, float *ang_var);

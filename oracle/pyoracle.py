@@ -70,6 +70,8 @@ _SIGS = {
     "orc_splay_divergence": (ctypes.c_double, [P, F, I]),
     "orc_thermal_run": (ctypes.c_int, [P, F, I, F, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int,
                                        ctypes.c_int, ctypes.c_uint64, ctypes.c_int, D, ctypes.c_int]),
+    "orc_electric_run": (ctypes.c_int, [P, F, I, F, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                        ctypes.c_int, ctypes.c_uint64, ctypes.c_int, D, ctypes.c_int]),
     "orc_time_sweeps": (ctypes.c_double, [P, F, I, F, ctypes.c_int, ctypes.c_uint64, ctypes.c_int, F]),
     "orc_sweep_points": (ctypes.c_int, [ctypes.c_double, ctypes.c_double, ctypes.c_double]),
     "orc_sweep_value": (ctypes.c_double, [ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int,
@@ -257,6 +259,15 @@ def thermal_run(p, ni, pt, ns, Ti, Tf, dT, MCT, MCS, seed, nthreads=1):
     rows = np.zeros(6 * n)
     got = lib().orc_thermal_run(ctypes.byref(p), fp(ni), ip(pt), fp(ns) if ns is not None else None, Ti, Tf, dT,
                                 MCT, MCS, seed, nthreads, dp(rows), n)
+    return rows[:6 * got].reshape(got, 6)
+
+
+def electric_run(p, ni, pt, ns, Ei, Ef, dE, MCT, MCS, seed, nthreads=1):
+    """Field sweep at the fixed temperature p.T: rows (field, S, varS, E, varE, NaN) per field value."""
+    n = lib().orc_sweep_points(Ei, Ef, dE)
+    rows = np.zeros(6 * n)
+    got = lib().orc_electric_run(ctypes.byref(p), fp(ni), ip(pt), fp(ns) if ns is not None else None, Ei, Ef, dE,
+                                 MCT, MCS, seed, nthreads, dp(rows), n)
     return rows[:6 * got].reshape(got, 6)
 
 

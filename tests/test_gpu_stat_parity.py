@@ -1,4 +1,5 @@
-"""Statistical parity of the GPU sweep with the CPU oracle on the 16^3 Lebwohl-Lasher thermal run.
+"""Statistical parity of the GPU sweep with the CPU oracle: the 16^3 Lebwohl-Lasher thermal run and the GHRL
+Freedericksz slab under an electric-field sweep.
 
 Case: A = 1, 16x16x16 bulk, periodic x/y/z, random initial directors from the replica's seed, T = 1.2 -> 0.1 with
 dT = -0.02 (56 points), MCT = 500 thermalization and MCS = 200 measurement sweeps per point, 16 replicas
@@ -70,3 +71,44 @@ def test_c1_thermal_parity_16_replicas(mcl, orc):
                                                                       zs[worst], report)
     # the sampled physics: ordered at low T, disordered at high T
     assert gpu[:, -1, 1].mean() > 0.95 and gpu[:, 0, 1].mean() < 0.15
+
+
+def test_c3_electric_slab_parity_16_replicas(mcl, orc):
+    """GHRL (K11, K22, K33) = (1.0, 0.6, 1.4) 16x16x8 slab, homeotropic W = 1.5 on both surfaces, T = 0.5, field
+    along x with elecA = 0.25 swept 0 -> 2.0 in steps of 0.2, MCT = 300, MCS = 200, 16 replicas (the oracle's run is
+    tests/golden/c3_oracle.npz, tests/golden/make_c3_fixture.py). Same z criterion as the thermal case over
+    4 x 11 comparisons: at most 2 beyond 3 sigma (0.1 expected by chance), none beyond 5."""
+    import importlib.util
+    from helpers import lattice_from_orc
+    spec = importlib.util.spec_from_file_location("make_c3_fixture", os.path.join(HERE, "golden", "make_c3_fixture.py"))
+    fx = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(fx)
+    c = fx.CASE
+    ref = np.load(os.path.join(HERE, "golden", "c3_oracle.npz"))["rows"]  # (16, 11, 6): field S varS E varE -
+    seeds = list(range(1, ref.shape[0] + 1))
+    pt, ns = orc.geometry("slab", c["Nx"], c["Ny"], c["Nz"])
+    p = fx.case_params(orc)
+    lats = [lattice_from_orc(mcl, p, orc.ic_random(pt, s), pt, ns, T=c["T"], seed=s) for s in seeds]
+    f32 = lambda x: float(np.float32(x))
+    Ei, Ef, dE = f32(c["Ei"]), f32(c["Ef"]), f32(c["dE"])
+    nE = orc.sweep_points(Ei, Ef, dE)
+    gpu = np.zeros((len(seeds), nE, 5))
+    with mcl.Engine(lats) as eng:
+        for ie in range(nE):
+            E = float(np.float32(orc.sweep_value(Ei, Ef, dE, ie, nE)))
+            eng.set_field(E)
+            eng.sweeps(c["MCT"])
+            eng.reset_moments()
+            eng.sweeps(c["MCS"], 1)
+            for r in range(len(seeds)):
+                m = eng.moments(r)
+                S1, S2, E1, E2 = m["S"] / m["n"], m["S2"] / m["n"], m["E"] / m["n"], m["E2"] / m["n"]
+                gpu[r, ie] = (E, S1, S2 - S1 * S1, E1, E2 - E1 * E1)
+    assert np.allclose(gpu[0, :, 0], ref[0, :, 0], atol=1e-6)
+    names = ["S", "varS", "E", "varE"]
+    zs = np.stack([z_scores(gpu[:, :, k], ref[:, :, k]) for k in (1, 2, 3, 4)])
+    report = "; ".join("%s max|z| %.2f at field %.1f" % (names[k], np.abs(zs[k]).max(),
+                                                          ref[0, int(np.argmax(np.abs(zs[k]))), 0]) for k in range(4))
+    print(report)
+    assert int((np.abs(zs) > 3).sum()) <= 2, report
+    assert np.abs(zs).max() < 5, report

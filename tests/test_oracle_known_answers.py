@@ -404,3 +404,35 @@ def test_detailed_balance_independent_spins(orc):
     # with correlated samples chi2 is inflated by ~ the autocorrelation time (~2-4 sweeps here); a biased
     # acceptance rule (e.g. exp(+dE/T) or a missing factor) gives chi2 in the 1e4-1e5 range
     assert chi2 < 20 * 10, chi2
+
+
+# ---- electric mode loop (evolve_electric.cpp:31-103) ----
+
+def test_electric_run_zero_field_point_equals_thermal_point(orc):
+    """At field 0 the first point of a field sweep is the thermal point at the same T: same RNG seeding, ang_var
+    start and MCT/MCS schedule, and the field term vanishes, so the rows agree bit for bit."""
+    surf = [("homeotropic", 1.5, (0, 0, 0)), ("homeotropic", 1.5, (0, 0, 0))]
+    pt, ns = orc.geometry("slab", 8, 8, 6)
+    kw = dict(periodic=(1, 1, 0), potential="ghrl", k=(1.0, 0.6, 1.4, 0.0), elec=(1, 0, 0), elecA=0.25, elecE=0.0,
+              T=0.5, surfaces=surf)
+    p1, p2 = orc.params(8, 8, 6, **kw), orc.params(8, 8, 6, **kw)
+    n1, n2 = orc.ic_random(pt, 3), orc.ic_random(pt, 3)
+    th = orc.thermal_run(p1, n1, pt, ns, 0.5, 0.5, -0.1, 20, 10, 3, 1)
+    el = orc.electric_run(p2, n2, pt, ns, 0.0, 0.4, 0.2, 20, 10, 3, 1)
+    assert el.shape == (3, 6) and list(el[:, 0]) == [0.0, pytest.approx(0.2), pytest.approx(0.4)]
+    assert np.array_equal(th[0, 1:5], el[0, 1:5])
+    assert np.isnan(el[0, 5])
+
+
+def test_electric_run_aligns_with_a_strong_field(orc):
+    """LL bulk, dielectric coupling 1 along x: at field 2 the field energy -(2 n_x)^2 dominates T = 0.5, the directors
+    align along x and the mean site energy drops by about 4 below the zero-field value."""
+    pt, ns = orc.geometry("bulk", 6, 6, 6)
+    p = orc.params(6, 6, 6, potential="ll", elec=(1, 0, 0), elecA=1.0, elecE=0.0, T=0.5)
+    ni = orc.ic_random(pt, 4)
+    rows = orc.electric_run(p, ni, pt, ns, 0.0, 2.0, 2.0, 300, 100, 4, 1)
+    assert rows.shape[0] == 2
+    assert rows[1, 1] > 0.9
+    assert rows[0, 3] - rows[1, 3] > 3.0
+    lam, v, _ = orc.eig_sym3(orc.q_tensor(ni, pt))
+    assert abs(v[0]) > 0.99
