@@ -12,6 +12,15 @@ length of the benchmark configuration, after 2000 warmup sweeps; the directors s
 Prints one JSON line: site-updates/ns over the timed sweeps, the roofline of the sweep kernel (algorithmic bytes
 per launch over its mean launch duration, measured with HIP events on the library's stream) and the CPU baseline
 (the oracle's OpenMP restatement of the reference sweep, timed on this host's cores on a bounded sample).
+
+Two further workloads print a line of the same form (the default run prints only the 128^3 one):
+  --workload slab    GHRL (K11, K22, K33) = (1.0, 0.6, 1.4) 256x256x64 slab, periodic x/y, free z, homeotropic
+                     W = 1.5 on both z surfaces, T = 0.5, random directors, field along x (dielectric coupling
+                     0.25) swept 0 -> 2.0 in 40 steps, MCT = 200 and MCS = 100 sweeps per field value, each MCS
+                     sweep measured (electric mode); --warmup sweeps at field 0 first.
+  --workload batch   512 Lebwohl-Lasher lattices in one context, sizes 16^3..64^3, bulk or homeotropic sphere
+                     (W = 1.0), T uniform in [0.9, 1.3], per-lattice seeds, autotuned launch shape, MCT = 1000 and
+                     MCS = 2000 with a measurement every 10 sweeps (graph replay is off while kernels are timed).
 """
 import argparse
 import json
@@ -28,6 +37,7 @@ sys.path.insert(0, ROOT)
 METRIC = "Metropolis site-updates/ns at 128^3 LL; GPU sweeps to locate T_NI to ±0.002"
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
 BYTES_PER_UPDATE = 24          # read + write of one FP32 director triplet per site update
+BYTES_PER_MASKED_UPDATE = 25   # plus the one-byte point type of a masked lattice (slab, sphere)
 N = 128
 T_BENCH = 1.10
 MEASURE_EVERY = 10
@@ -66,6 +76,168 @@ def cpu_baseline(budget_s=15.0):
                       "threads" % (nsw, t, threads)}
 
 
+def kernel_roofline(launches, kms, bytes_per_launch, copy_gbs):
+    avg_launch_s = kms / launches / 1e3
+    achieved = bytes_per_launch / avg_launch_s / 1e9
+    return {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_sweep (one colour pass)",
+            "bytes_per_launch": bytes_per_launch, "avg_launch_us": round(avg_launch_s * 1e6, 3),
+            "timed_launches": launches, "copy_measured_gbs": None if copy_gbs is None else round(copy_gbs, 1)}
+
+
+def copy_check(nbytes):
+    from mclics_amd import capi
+    if nbytes <= 0:
+        return None
+    gbs, _ = capi.copy_bandwidth(nbytes, 20)
+    log("bench: device copy of %d MiB: %.0f GB/s (read + write)" % (nbytes >> 20, gbs))
+    return gbs
+
+
+def line(value, steps, warmup, elapsed, config, roofline, cpu, dtype="f32"):
+    return {"metric": METRIC, "value": round(value, 4), "unit": "site-updates/ns", "n_gpus": 1, "steps": steps,
+            "warmup": warmup, "ms_per_step": round(elapsed * 1e3 / steps, 6), "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": dtype, "data": "synthetic", "config": config,
+            "roofline": roofline, "cpu_baseline": cpu}
+
+
+SLAB = dict(dims=(256, 256, 64), k=(1.0, 0.6, 1.4), W=1.5, T=0.5, elecA=0.25, Ei=0.0, Ef=2.0, steps=40, MCT=200,
+            MCS=100, seed=1)
+
+
+def slab_model(capi):
+    """The slab workload's physics through libmclgpu (GHRL coefficients of setGHRL, io.cpp:340-360)."""
+    c = capi.ghrl_coefficients(*SLAB["k"], 0.0)
+    return capi.make_model("ghrl", ghrl=(c["lam"], c["mu"], c["nu"], c["rho"], c["sigma"]), elec_dir=(1.0, 0.0, 0.0),
+                           elecA=SLAB["elecA"], elecE=0.0)
+
+
+def run_slab(a):
+    from mclics_amd import batch, capi
+    nx, ny, nz = SLAB["dims"]
+    pt, ns = batch.slab_geometry(nx, ny, nz)
+    nvalid = int((pt > 0).sum())
+    lat = capi.Lattice(nx, ny, nz, batch.random_directors(pt, SLAB["seed"]), periodic=(1, 1, 0), pt=pt, ns=ns,
+                       surfaces=[("homeotropic", SLAB["W"], (0, 0, 0)), ("homeotropic", SLAB["W"], (0, 0, 0))],
+                       model=slab_model(capi), T=SLAB["T"], seed=SLAB["seed"])
+    fields = [float(np.float32(SLAB["Ei"] + (SLAB["Ef"] - SLAB["Ei"]) * i / SLAB["steps"]))
+              for i in range(SLAB["steps"] + 1)]
+    per = SLAB["MCT"] + SLAB["MCS"]
+    steps = len(fields) * per
+    log("bench: slab %dx%dx%d GHRL, %d field values, %d sweeps" % (nx, ny, nz, len(fields), steps))
+    rows = []
+    with capi.Engine(lat) as eng:
+        eng.sweeps(a.warmup)
+        eng.synchronize()
+        eng.set_kernel_timing(a.timing_stride, (steps + a.timing_stride - 1) // a.timing_stride + len(fields) * 2)
+        t0 = time.perf_counter()
+        for E in fields:
+            eng.set_field(E)
+            eng.sweeps(SLAB["MCT"])
+            eng.reset_moments()
+            eng.sweeps(SLAB["MCS"], 1)
+            m = eng.moments()
+            rows.append((E, m["S"] / m["n"], m["E"] / m["n"]))
+        eng.synchronize()
+        t1 = time.perf_counter()
+        launches, kms = eng.kernel_timing()
+        eng.set_kernel_timing(0)
+    elapsed = t1 - t0
+    value = steps * nvalid / (elapsed * 1e9)
+    roof = kernel_roofline(launches, kms, (nvalid // 2) * BYTES_PER_MASKED_UPDATE, copy_check(a.copy_bytes))
+    log("bench: %.3f s, %.2f site-updates/ns; E(field 0) = %.4f, E(field 2) = %.4f" % (elapsed, value, rows[0][2],
+                                                                                      rows[-1][2]))
+    cpu = None if a.no_cpu_baseline else cpu_baseline_slab()
+    cfg = {"workload": "GHRL (1.0, 0.6, 1.4) 256x256x64 slab, periodic x/y, homeotropic W=1.5 on both z "
+                       "surfaces, T=0.5, field along x (elecA 0.25) swept 0->2.0 in 40 steps, MCT 200 + MCS 100 "
+                       "per field, every MCS sweep measured",
+           "lattice": list(SLAB["dims"]), "fields": len(fields), "sweeps": steps,
+           "S_E": [[round(x, 5) for x in r] for r in rows[::10]]}
+    return line(value, steps, a.warmup, elapsed, cfg, roof, cpu)
+
+
+def cpu_baseline_slab(budget_s=15.0):
+    """The oracle's OpenMP sweep on the same slab (same physics, field 1.0), bounded to about budget_s seconds."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as orc
+    threads = max(1, min(int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1, os.cpu_count() or 1))
+    nx, ny, nz = SLAB["dims"]
+    pt, ns = orc.geometry("slab", nx, ny, nz)
+    p = orc.params(nx, ny, nz, periodic=(1, 1, 0), potential="ghrl", k=SLAB["k"] + (0.0,), elec=(1, 0, 0),
+                   elecA=SLAB["elecA"], elecE=1.0, T=SLAB["T"],
+                   surfaces=[("homeotropic", SLAB["W"], (0, 0, 0)), ("homeotropic", SLAB["W"], (0, 0, 0))])
+    ni = orc.ic_random(pt, 1)
+    t1, av = orc.time_sweeps(p, ni, pt, ns, 2, 1, threads)
+    nsw = max(3, int(budget_s / max(t1 / 2, 1e-3)))
+    t, _ = orc.time_sweeps(p, ni, pt, ns, nsw, 2, threads, av)
+    rate = nsw * nx * ny * nz / (t * 1e9)
+    log("bench: CPU %.4f site-updates/ns on %d threads" % (rate, threads))
+    return {"value": rate, "unit": "site-updates/ns", "cores": threads, "kind": "port",
+            "sample": "%d sweeps of the same slab at field 1.0 (%.1f s), oracle OpenMP sweep with %d threads" % (
+                nsw, t, threads)}
+
+
+def run_batch_workload(a):
+    from mclics_amd import batch, capi
+    specs = batch.c4_specs(512, 1)
+    lats = [batch.build_lattice(s) for s in specs]
+    nvalid = np.array([int((l.pt > 0).sum()) if l.pt is not None else int(np.prod(l.dims)) for l in lats])
+    mct, mcs, every = 1000, 2000, 10
+    steps = mct + mcs
+    log("bench: batch of %d lattices, %d valid sites, %d sweeps" % (len(lats), nvalid.sum(), steps))
+    with capi.Engine(lats) as eng:
+        tune = eng.autotune(10)
+        eng.sweeps(a.warmup)
+        eng.synchronize()
+        eng.set_kernel_timing(a.timing_stride, steps // a.timing_stride + 2 * (mcs // every) + 4)
+        t0 = time.perf_counter()
+        eng.sweeps(mct)
+        eng.reset_moments()
+        eng.sweeps(mcs, every)
+        eng.synchronize()
+        t1 = time.perf_counter()
+        launches, kms = eng.kernel_timing()
+        eng.set_kernel_timing(0)
+        S = np.array([eng.moments(r)["S"] / eng.moments(r)["n"] for r in range(len(lats))])
+    elapsed = t1 - t0
+    value = steps * float(nvalid.sum()) / (elapsed * 1e9)
+    masked_sites = float(sum(n for n, l in zip(nvalid, lats) if l.pt is not None))
+    bulk_sites = float(nvalid.sum()) - masked_sites
+    bytes_per_launch = int((bulk_sites * BYTES_PER_UPDATE + masked_sites * BYTES_PER_MASKED_UPDATE) / 2)
+    roof = kernel_roofline(launches, kms, bytes_per_launch, copy_check(a.copy_bytes))
+    log("bench: %.3f s, %.2f site-updates/ns, launch shape %s" % (elapsed, value, tune))
+    cfg = {"workload": "512 LL lattices, sizes 16^3..64^3 bulk or homeotropic sphere (W=1.0), T uniform in "
+                       "[0.9, 1.3], per-lattice seeds, MCT 1000 + MCS 2000, measured every 10 sweeps",
+           "lattices": len(lats), "valid_sites": int(nvalid.sum()), "sweeps": steps, "launch": tune,
+           "S_range": [round(float(S.min()), 4), round(float(S.max()), 4)]}
+    cpu = None if a.no_cpu_baseline else cpu_baseline_batch(specs[:16])
+    return line(value, steps, a.warmup, elapsed, cfg, roof, cpu)
+
+
+def cpu_baseline_batch(specs, sweeps=20):
+    """The oracle's OpenMP sweep on the first lattices of the batch (same geometry, physics and temperature),
+    `sweeps` sweeps each."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as orc
+    threads = max(1, min(int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1, os.cpu_count() or 1))
+    tot_t, tot_u = 0.0, 0.0
+    for s in specs:
+        L = s["L"]
+        pt, ns = orc.geometry(s["geometry"], L, L, L)
+        surf = [("homeotropic", 1.0, (0, 0, 0))] if s["geometry"] == "sphere" else []
+        per = (0, 0, 0) if s["geometry"] == "sphere" else (1, 1, 1)
+        p = orc.params(L, L, L, periodic=per, potential="ll", A=1.0, T=s["T"], surfaces=surf)
+        ni = orc.ic_random(pt, s["seed"])
+        t, _ = orc.time_sweeps(p, ni, pt, ns if surf else None, sweeps, s["seed"], threads)
+        tot_t += t
+        tot_u += sweeps * float((pt > 0).sum())
+    rate = tot_u / (tot_t * 1e9)
+    log("bench: CPU %.4f site-updates/ns on %d threads" % (rate, threads))
+    return {"value": rate, "unit": "site-updates/ns", "cores": threads, "kind": "port",
+            "sample": "%d sweeps each of the first %d lattices of the batch (%.1f s), oracle OpenMP sweep with %d "
+                      "threads" % (sweeps, len(specs), tot_t, threads)}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -75,6 +247,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-bytes", type=int, default=256 << 20,
                     help="size of the STREAM-style copy that checks the HBM peak (0: skip)")
+    ap.add_argument("--workload", choices=("ll128", "slab", "batch"), default="ll128")
     a = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if a.gpus != 1 or world != 1:
@@ -84,6 +257,10 @@ def main():
 
     from mclics_amd import capi
     capi.load()
+    if a.workload != "ll128":
+        out = run_slab(a) if a.workload == "slab" else run_batch_workload(a)
+        print(json.dumps(out), flush=True)
+        return
     nt = N ** 3
     ni = np.tile(np.array([0.0, 0.0, 1.0], dtype=np.float32), nt)
     lat = capi.Lattice(N, N, N, ni, periodic=(1, 1, 1), model=capi.make_model("ll", A=1.0), T=T_BENCH,

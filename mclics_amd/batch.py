@@ -45,6 +45,20 @@ def sphere_geometry(n):
     return pt, ns
 
 
+@functools.lru_cache(maxsize=16)
+def slab_geometry(nx, ny, nz):
+    """Point types and outward normals of the slab geometry (geometry_slab.cpp:51-71): surface 0 (type 2, normal
+    -z) on the plane k = 0, surface 1 (type 3, normal +z) on k = Nz - 1, bulk between. Cached; read-only."""
+    pt = np.ones((nz, ny, nx), dtype=np.int32)
+    ns = np.zeros((nz, ny, nx, 3), dtype=np.float32)
+    pt[0], pt[-1] = 2, 3
+    ns[0, ..., 2], ns[-1, ..., 2] = -1.0, 1.0
+    pt, ns = pt.reshape(-1), ns.reshape(-1)
+    pt.flags.writeable = False
+    ns.flags.writeable = False
+    return pt, ns
+
+
 def random_directors(pt, seed):
     """Isotropic unit vectors on non-empty sites, (0, 0, 1) on empty ones."""
     rng = np.random.default_rng(seed)

@@ -8,6 +8,7 @@ energy_calculator_GPU + Q_tensor + Eigen_value_evaluation (src/evolve.cu:442-478
 Loading fails loudly when the library has not been built: there is no CPU fallback.
 """
 import ctypes
+import math
 import os
 
 import numpy as np
@@ -144,6 +145,21 @@ def _check(code):
 
 def _fptr(a):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def ghrl_coefficients(k11, k22, k33, p0=0.0):
+    """GHRL coefficients from the Frank constants (setGHRL, io.cpp:340-360), in the reference's FP32 arithmetic:
+    Scale = |9 / (k11 - 3 k22 - k33)| so that |nu| = 1, lambda = (Scale/9)(2 k11 - 3 k22 + k33),
+    mu = Scale (k22 - k11), rho = (Scale/9)(k11 - k33), nu = (Scale/9)(k11 - 3 k22 - k33),
+    sigma = -Scale k22 2 pi / p0 (0 without chirality)."""
+    f = np.float32
+    k11, k22, k33, p0 = f(k11), f(k22), f(k33), f(p0)
+    scale = f(abs(9.0 / float(k11 - f(3) * k22 - k33)))
+    s9 = scale / f(9)
+    sigma = f(-float(scale) * float(k22) * (2 * math.pi / float(p0))) if p0 else f(0)
+    return {"lam": float(s9 * (f(2) * k11 - f(3) * k22 + k33)), "mu": float(scale * (k22 - k11)),
+            "nu": float(s9 * (k11 - f(3) * k22 - k33)), "rho": float(s9 * (k11 - k33)), "sigma": float(sigma),
+            "scale": float(scale)}
 
 
 def make_model(potential="ll", A=1.0, B1=0.04, C=0.3, ghrl=(0.0, 0.0, 1.0, 0.0, 0.0), elec_dir=(0.0, 0.0, 0.0),

@@ -55,7 +55,7 @@ def features(T, A, W, field, nvalid):
                      np.full(T.shape, field, dtype=np.float64), np.full(T.shape, np.log(nvalid))], axis=-1)
 
 
-def generate(runs=25, lattices=512, rungs=16, mct=50, mcs=150, every=2, seed=1, lmin=16, lmax=64, tmin=0.9,
+def generate(runs=25, lattices=512, rungs=16, mct=250, mcs=750, every=3, seed=1, lmin=16, lmax=64, tmin=0.9,
              tmax=1.3, log=log):
     """Pairs (features, observables) from `runs` batched runs. Each lattice walks a ladder of `rungs` temperatures
     drawn uniformly in [tmin A, tmax A] (downwards from random directors or upwards from directors along z),
@@ -227,9 +227,9 @@ def main(argv=None):
     g.add_argument("--runs", type=int, default=25)
     g.add_argument("--lattices", type=int, default=512)
     g.add_argument("--rungs", type=int, default=16)
-    g.add_argument("--mct", type=int, default=50)
-    g.add_argument("--mcs", type=int, default=150)
-    g.add_argument("--every", type=int, default=2)
+    g.add_argument("--mct", type=int, default=250)
+    g.add_argument("--mcs", type=int, default=750)
+    g.add_argument("--every", type=int, default=3)
     g.add_argument("--seed", type=int, default=1)
     g.add_argument("--out", required=True)
     t = sub.add_parser("train")

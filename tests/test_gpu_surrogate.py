@@ -3,8 +3,9 @@
 A network is trained with the HIP kernels on pairs from batched plain ladders of 8^3..16^3 lattices, then
   - placement: its predicted susceptibility peak for a 12^3 bulk lattice lies near the peak of a dense plain scan,
     where an untrained network (the same architecture and normalization, initial weights) puts it far away;
-  - warm start: at T = 1.0 a 32^3 lattice started from the network's predicted order is within a few per mille of
-    the equilibrium energy after 100 sweeps, where the reference's random start is not.
+  - warm start: at T = 1.0 a 32^3 lattice started from the network's predicted order has an equilibrium energy
+    (within 4 standard deviations of the equilibrium fluctuation) after 100 sweeps, where the reference's random
+    start is still more than 10 standard deviations away.
 Every number compared here is a plain Metropolis average; the network only chooses temperatures and starts.
 """
 import numpy as np
@@ -77,8 +78,9 @@ def test_warm_start_gain(trained):
         eng.sweeps(1000, 5)
         m = eng.moments(4)
         E_eq = m["E"] / m["n"]
-    print("S0 %.3f  E_eq %.4f  E(100): random %s warm %s" % (S0, E_eq, E100[:2], E100[2:4]))
+        sd = (m["E2"] / m["n"] - E_eq * E_eq) ** 0.5
+    print("S0 %.3f  E_eq %.4f +- %.4f  E(100): random %s warm %s" % (S0, E_eq, sd, E100[:2], E100[2:4]))
     cold = min(abs(E100[0] - E_eq), abs(E100[1] - E_eq))
     warm = max(abs(E100[2] - E_eq), abs(E100[3] - E_eq))
-    assert warm < 0.005 * abs(E_eq)
-    assert warm < 0.25 * cold
+    assert warm < 4 * sd
+    assert cold > 10 * sd
