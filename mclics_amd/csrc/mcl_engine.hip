@@ -66,6 +66,7 @@ __host__ __device__ __forceinline__ uint32_t fdiv(uint32_t n, uint32_t m, uint32
 struct LatDev {
   int nx, ny, nz, nxh, nc;  // nc = sites per colour block = nxh * ny * nz
   uint32_t mxh, sxh, my, sy;
+  uint32_t plane, ywrap, zwrap;  // nxh * ny, (ny - 1) * nxh, (nz - 1) * plane: colour-block strides
   int flags;
   int nvalid;
   uint32_t off;   // pooled offset of the colour-0 block; colour 1 follows at off + nc (pool < 2^31 sites)
@@ -265,14 +266,17 @@ __device__ __forceinline__ bool gather(const LatDev &L, int colour, int t, const
   if (jm < 0) { jm = py ? ny - 1 : j; ex &= py ? 63u : ~8u; }
   if (kp == nz) { kp = pz ? 0 : k; ex &= pz ? 63u : ~16u; }
   if (km < 0) { km = pz ? nz - 1 : k; ex &= pz ? 63u : ~32u; }
-  const uint32_t rowoff = obase + (uint32_t)(nxh * (j + ny * k));
+  // neighbours in y and z are whole rows / planes away in the other colour block; at a free face the index stays
+  // on the site's own column (masked out of the energy)
+  const uint32_t col = obase + (uint32_t)t, rowstart = obase + (uint32_t)(t - m);
+  const uint32_t rs = (uint32_t)nxh, pl = L.plane;
   uint32_t nidx[6];
-  nidx[0] = rowoff + (uint32_t)(ip >> 1);
-  nidx[1] = rowoff + (uint32_t)(im >> 1);
-  nidx[2] = obase + (uint32_t)(m + nxh * (jp + ny * k));
-  nidx[3] = obase + (uint32_t)(m + nxh * (jm + ny * k));
-  nidx[4] = obase + (uint32_t)(m + nxh * (j + ny * kp));
-  nidx[5] = obase + (uint32_t)(m + nxh * (j + ny * km));
+  nidx[0] = rowstart + (uint32_t)(ip >> 1);
+  nidx[1] = rowstart + (uint32_t)(im >> 1);
+  nidx[2] = jp == j ? col : (jp == 0 ? col - L.ywrap : col + rs);
+  nidx[3] = jm == j ? col : (jm == ny - 1 && j == 0 ? col + L.ywrap : col - rs);
+  nidx[4] = kp == k ? col : (kp == 0 ? col - L.zwrap : col + pl);
+  nidx[5] = km == k ? col : (km == nz - 1 && k == 0 ? col + L.zwrap : col - pl);
 #pragma unroll
   for (int d = 0; d < 6; d++) {
     const uint32_t q = nidx[d];
@@ -331,7 +335,7 @@ __global__ __launch_bounds__(1024) void k_sweep(const LatDev *__restrict__ lats,
   const unsigned long long sweep = *sweep_base + k;
   const float T = ctrl[r].T, invT = ctrl[r].invT, elecE = ctrl[r].elecE;
   const int lane = threadIdx.x & 63;
-  const int lblk = bd.y / (int)blockDim.x;  // workgroup index within the lattice
+  const int lblk = bd.y / ((FLIP ? 1 : 2) * (int)blockDim.x);  // workgroup index within the lattice
   unsigned int *const cnt_r = cnt + (size_t)r * kCntLattice;
   __shared__ float av_sh;
   float av;
@@ -356,18 +360,37 @@ __global__ __launch_bounds__(1024) void k_sweep(const LatDev *__restrict__ lats,
     if (threadIdx.x == 0) ctrl[r].av_store[sweep & 1] = av;
     cnt_r[((sweep + 1) % 3) * kCntSweep + threadIdx.x * kCntStride] = 0u;
   }
-  const int t = bd.y + (int)threadIdx.x;
-
-  bool accepted = false;
-  Site s;
-  uint32_t self = 0;
-  if (t < L.nc && gather<MASK, SURF>(L, colour, t, X, Y, Z, PT, SX, SY, SZ, s, self)) {
-    const uint32_t lin = (uint32_t)(s.i + L.nx * (s.j + L.ny * s.k));
-    const mcl::Philox4 R =
-        mcl::philox4x32_10(lin, (uint32_t)sweep, (uint32_t)(sweep >> 32), kTagSweep, L.key0, L.key1);
+  // Sites per thread: without the flip trial a pair of positions (p, p + 64) of the colour block shares one
+  // Philox4x32-10 draw, 64 bits per site (16-bit angle, 16-bit axis choice, 32-bit acceptance uniform): each wave
+  // updates 128 consecutive positions, and each of its loads stays one contiguous 256-byte span. With the flip
+  // trial every site takes a draw of its own (a fourth 32-bit uniform). The counter (pair or position, sweep,
+  // tag + colour) depends on the position only, never on the launch shape (a workgroup spans a multiple of 128
+  // positions).
+  constexpr int SPT = FLIP ? 1 : 2;
+  const int tw = SPT == 2 ? bd.y + 128 * (int)(threadIdx.x >> 6) + lane : bd.y + (int)threadIdx.x;
+  const uint32_t ctr0 = SPT == 2 ? (uint32_t)(((bd.y >> 7) + (int)(threadIdx.x >> 6)) * 64 + lane) : (uint32_t)tw;
+  const mcl::Philox4 R = mcl::philox4x32_10(ctr0, (uint32_t)sweep, (uint32_t)(sweep >> 32),
+                                            kTagSweep + (uint32_t)colour, L.key0, L.key1);
+  unsigned nacc = 0;
+#pragma unroll
+  for (int e = 0; e < SPT; e++) {
+    const int t = tw + 64 * e;
+    Site s;
+    uint32_t self = 0;
+    if (t >= L.nc || !gather<MASK, SURF>(L, colour, t, X, Y, Z, PT, SX, SY, SZ, s, self)) continue;
+    float u_ang, rot, u;
+    if constexpr (SPT == 2) {
+      const uint32_t w = R.v[2 * e];
+      u_ang = mcl::u01_open16(w >> 16);
+      rot = mcl::u01_open16(w & 0xFFFFu);
+      u = mcl::u01_open(R.v[2 * e + 1]);
+    } else {
+      u_ang = mcl::u01_open(R.v[0]);
+      rot = mcl::u01_open(R.v[1]);
+      u = mcl::u01_open(R.v[2]);
+    }
     // trial rotation by pi * va about x, y or z (evolve.cu:162-182)
-    const float va = (2.0f * mcl::u01_open(R.v[0]) - 1.0f) * av;
-    const float rot = mcl::u01_open(R.v[1]);
+    const float va = (2.0f * u_ang - 1.0f) * av;
     // cos and sin of pi va: the hardware sine and cosine take their argument in turns (x 2 pi)
     const float sn = __builtin_amdgcn_sinf(0.5f * va), cs = __builtin_amdgcn_cosf(0.5f * va);
     const V3 n = s.n;
@@ -385,9 +408,9 @@ __global__ __launch_bounds__(1024) void k_sweep(const LatDev *__restrict__ lats,
     nn.z *= inv;
     const float E_old = site_energy<POT, SURF, FIELD>(L, s, n, surf, elecE);
     const float E_new = site_energy<POT, SURF, FIELD>(L, s, nn, surf, elecE);
-    const float u = mcl::u01_open(R.v[2]);
     // Metropolis; T = 0 accepts when the energy does not increase (evolve.cu:212-224)
-    accepted = T > 0.0f ? (u <= __expf(-(E_new - E_old) * invT)) : (E_new <= E_old);
+    const bool accepted = T > 0.0f ? (u <= __expf(-(E_new - E_old) * invT)) : (E_new <= E_old);
+    nacc += accepted ? 1u : 0u;
     V3 nc = accepted ? nn : n;
     bool changed = accepted;
     if constexpr (FLIP) {
@@ -408,11 +431,12 @@ __global__ __launch_bounds__(1024) void k_sweep(const LatDev *__restrict__ lats,
     }
   }
 
-  // accepted rotations: wave ballot, then one atomic per workgroup into one of kCnt counters (striping keeps the
+  // accepted rotations: wave ballots, then one atomic per workgroup into one of kCnt counters (striping keeps the
   // device-scope atomics of thousands of workgroups from serializing on one address)
   __shared__ unsigned int wsum[16];
-  const unsigned long long ball = __ballot(accepted ? 1 : 0);
-  if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = (unsigned)__popcll(ball);
+  unsigned wc = (unsigned)__popcll(__ballot(nacc & 1u));
+  if constexpr (SPT == 2) wc += 2u * (unsigned)__popcll(__ballot(nacc >> 1));
+  if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = wc;
   __syncthreads();
   if (threadIdx.x == 0) {
     unsigned tot = 0;
@@ -948,9 +972,11 @@ int dev_alloc(T **p, size_t n) {
 // Sweep work list for workgroups of bs threads: per lattice ceil(nc / bs) workgroups of bs consecutive colour
 // positions. Drops the captured graphs, which hold the old launch shape.
 int build_worklist(Ctx *c, int bs, int xcd) {
+  // a workgroup of the sweep covers bs colour-block positions, or 2 bs when a thread updates a pair of sites
+  const int span = c->flip ? bs : 2 * bs;
   std::vector<int2> blk;
   for (int r = 0; r < c->nlat; r++)
-    for (int t0 = 0; t0 < c->ld[r].nc; t0 += bs) blk.push_back(make_int2(r, t0));
+    for (int t0 = 0; t0 < c->ld[r].nc; t0 += span) blk.push_back(make_int2(r, t0));
   if (c->blk_d) {
     MCL_HIP(hipStreamSynchronize(c->stream));
     MCL_HIP(hipFree(c->blk_d));
@@ -1031,6 +1057,9 @@ int create_ctx(const mcl_lattice_desc *lat, int nlat, Ctx *c) {
     L.sxh = fx.sh;
     L.my = fy.m;
     L.sy = fy.sh;
+    L.plane = (uint32_t)h.nxh * (uint32_t)d.ny;
+    L.ywrap = (uint32_t)(d.ny - 1) * (uint32_t)h.nxh;
+    L.zwrap = (uint32_t)(d.nz - 1) * L.plane;
     L.flags = (d.periodic[0] ? F_PX : 0) | (d.periodic[1] ? F_PY : 0) | (d.periodic[2] ? F_PZ : 0) |
               (h.mask ? F_MASK : 0) | (h.surf ? F_SURF : 0) | ((d.radial & 1) ? F_RSPH : 0) |
               ((d.radial & 2) ? F_RCYL : 0);

@@ -90,12 +90,14 @@ def test_autotune_keeps_the_state(mcl):
 
 def test_small_batch_physics(mcl):
     """A small version of the batched configuration: ordered lattices at low T, disordered at high T, FP64 moments
-    and chi from every lattice."""
+    and chi from every lattice. The lattices are quenched from random directors for 600 sweeps, so a small periodic
+    lattice may still hold a defect: the test separates ordered from disordered, the equilibrium values are the
+    statistical parity tests' business."""
     specs = batch.c4_specs(24, seed=3, lmin=10, lmax=16, tmin=0.5, tmax=1.8)
     rows, info = batch.run_batch(specs, 300, 300, every=5, autotune=True, graph=16)
     T, S = rows[:, 0], rows[:, 1]
     bulk = np.array([s["geometry"] == "bulk" for s in specs])
-    assert np.all(S[bulk & (T < 0.8)] > 0.7)
+    assert np.all(S[bulk & (T < 0.8)] > 0.5)
     assert np.all(S[bulk & (T > 1.5)] < 0.4)
     assert np.all(rows[:, 2] >= 0) and np.all(rows[:, 4] >= 0) and np.all(np.isfinite(rows))
     assert info["site_updates_per_ns"] > 0
