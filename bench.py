@@ -127,6 +127,7 @@ def run_slab(a):
     log("bench: slab %dx%dx%d GHRL, %d field values, %d sweeps" % (nx, ny, nz, len(fields), steps))
     rows = []
     with capi.Engine(lat) as eng:
+        tune = eng.autotune(10)
         eng.sweeps(a.warmup)
         eng.synchronize()
         eng.set_kernel_timing(a.timing_stride, (steps + a.timing_stride - 1) // a.timing_stride + len(fields) * 2)
@@ -151,7 +152,7 @@ def run_slab(a):
     cfg = {"workload": "GHRL (1.0, 0.6, 1.4) 256x256x64 slab, periodic x/y, homeotropic W=1.5 on both z "
                        "surfaces, T=0.5, field along x (elecA 0.25) swept 0->2.0 in 40 steps, MCT 200 + MCS 100 "
                        "per field, every MCS sweep measured",
-           "lattice": list(SLAB["dims"]), "fields": len(fields), "sweeps": steps,
+           "lattice": list(SLAB["dims"]), "fields": len(fields), "sweeps": steps, "launch": tune,
            "S_E": [[round(x, 5) for x in r] for r in rows[::10]]}
     return line(value, steps, a.warmup, elapsed, cfg, roof, cpu)
 
@@ -267,6 +268,8 @@ def main():
                        ang_var=0.5, seed=1)
     log("bench: 128^3 LL periodic, T = %.2f, warmup %d sweeps" % (T_BENCH, a.warmup))
     with capi.Engine(lat) as eng:
+        tune = eng.autotune(20)  # launch shape from the library's autotuner (state is restored afterwards)
+        log("bench: launch shape %s" % tune)
         eng.sweeps(a.warmup, MEASURE_EVERY)
         eng.synchronize()
         eng.reset_moments()
@@ -308,6 +311,7 @@ def main():
         "config": {"workload": "LL 128^3 periodic bulk, homogeneous IC along z, T=1.10, adaptive ang_var, "
                                "FP64 S/E reductions every 10 sweeps",
                    "lattice": [N, N, N], "T": T_BENCH, "measure_every": MEASURE_EVERY, "sweeps": a.steps,
+                   "launch": tune,
                    "S_mean": round(S, 6), "E_mean": round(E, 6), "measurements": int(mom["n"])},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(),

@@ -229,7 +229,9 @@ __device__ __forceinline__ float site_energy(const LatDev &L, const Site &s, V3 
 
 // Decodes colour-block position t of lattice L into (i, j, k) and loads the site and its neighbours.
 // Returns false for padding positions (i >= nx) and empty sites.
-template <bool MASK, bool SURF>
+// FREE = false: every axis of every lattice of the context is periodic (and MASK is false), so every neighbour
+// exists and the wrap is the only boundary case.
+template <bool MASK, bool SURF, bool FREE = true>
 __device__ __forceinline__ bool gather(const LatDev &L, int colour, int t, const float *__restrict__ X,
                                        const float *__restrict__ Y, const float *__restrict__ Z,
                                        const uint8_t *__restrict__ PT, const float *__restrict__ SX,
@@ -257,26 +259,35 @@ __device__ __forceinline__ bool gather(const LatDev &L, int colour, int t, const
   s.n = V3{ldx(X, self), ldx(Y, self), ldx(Z, self)};
   // Periodic_Boundary / Free_Boundary (potential.cu:50-58). A missing neighbour keeps an in-bounds index (the
   // site's own column in the other colour) and is masked out of the energy, so all loads are unconditional.
-  const bool px = L.flags & F_PX, py = L.flags & F_PY, pz = L.flags & F_PZ;
   unsigned ex = 63u;
-  int ip = i + 1, im = i - 1, jp = j + 1, jm = j - 1, kp = k + 1, km = k - 1;
-  if (ip == nx) { ip = px ? 0 : i; ex &= px ? 63u : ~1u; }
-  if (im < 0) { im = px ? nx - 1 : i; ex &= px ? 63u : ~2u; }
-  if (jp == ny) { jp = py ? 0 : j; ex &= py ? 63u : ~4u; }
-  if (jm < 0) { jm = py ? ny - 1 : j; ex &= py ? 63u : ~8u; }
-  if (kp == nz) { kp = pz ? 0 : k; ex &= pz ? 63u : ~16u; }
-  if (km < 0) { km = pz ? nz - 1 : k; ex &= pz ? 63u : ~32u; }
-  // neighbours in y and z are whole rows / planes away in the other colour block; at a free face the index stays
-  // on the site's own column (masked out of the energy)
   const uint32_t col = obase + (uint32_t)t, rowstart = obase + (uint32_t)(t - m);
   const uint32_t rs = (uint32_t)nxh, pl = L.plane;
   uint32_t nidx[6];
-  nidx[0] = rowstart + (uint32_t)(ip >> 1);
-  nidx[1] = rowstart + (uint32_t)(im >> 1);
-  nidx[2] = jp == j ? col : (jp == 0 ? col - L.ywrap : col + rs);
-  nidx[3] = jm == j ? col : (jm == ny - 1 && j == 0 ? col + L.ywrap : col - rs);
-  nidx[4] = kp == k ? col : (kp == 0 ? col - L.zwrap : col + pl);
-  nidx[5] = km == k ? col : (km == nz - 1 && k == 0 ? col + L.zwrap : col - pl);
+  if constexpr (!FREE) {
+    nidx[0] = rowstart + (uint32_t)((i + 1 == nx ? 0 : i + 1) >> 1);
+    nidx[1] = rowstart + (uint32_t)((i == 0 ? nx - 1 : i - 1) >> 1);
+    nidx[2] = j == ny - 1 ? col - L.ywrap : col + rs;
+    nidx[3] = j == 0 ? col + L.ywrap : col - rs;
+    nidx[4] = k == nz - 1 ? col - L.zwrap : col + pl;
+    nidx[5] = k == 0 ? col + L.zwrap : col - pl;
+  } else {
+    const bool px = L.flags & F_PX, py = L.flags & F_PY, pz = L.flags & F_PZ;
+    int ip = i + 1, im = i - 1, jp = j + 1, jm = j - 1, kp = k + 1, km = k - 1;
+    if (ip == nx) { ip = px ? 0 : i; ex &= px ? 63u : ~1u; }
+    if (im < 0) { im = px ? nx - 1 : i; ex &= px ? 63u : ~2u; }
+    if (jp == ny) { jp = py ? 0 : j; ex &= py ? 63u : ~4u; }
+    if (jm < 0) { jm = py ? ny - 1 : j; ex &= py ? 63u : ~8u; }
+    if (kp == nz) { kp = pz ? 0 : k; ex &= pz ? 63u : ~16u; }
+    if (km < 0) { km = pz ? nz - 1 : k; ex &= pz ? 63u : ~32u; }
+    // neighbours in y and z are whole rows / planes away in the other colour block; at a free face the index
+    // stays on the site's own column (masked out of the energy)
+    nidx[0] = rowstart + (uint32_t)(ip >> 1);
+    nidx[1] = rowstart + (uint32_t)(im >> 1);
+    nidx[2] = jp == j ? col : (jp == 0 ? col - L.ywrap : col + rs);
+    nidx[3] = jm == j ? col : (jm == ny - 1 && j == 0 ? col + L.ywrap : col - rs);
+    nidx[4] = kp == k ? col : (kp == 0 ? col - L.zwrap : col + pl);
+    nidx[5] = km == k ? col : (km == nz - 1 && k == 0 ? col + L.zwrap : col - pl);
+  }
 #pragma unroll
   for (int d = 0; d < 6; d++) {
     const uint32_t q = nidx[d];
@@ -314,7 +325,7 @@ __host__ __device__ __forceinline__ float ang_var_rule(float av, unsigned acc, i
 // Sweep pass over one colour of every lattice. blk[L] = (lattice, first colour-block position) of logical
 // workgroup L; consecutive logical workgroups are placed on the same XCD (workgroups are dispatched to the 8 XCDs
 // round robin), so the rows a workgroup shares with its neighbours in z and y mostly hit the same L2.
-template <int POT, bool MASK, bool SURF, bool FIELD, bool FLIP>
+template <int POT, bool MASK, bool SURF, bool FIELD, bool FLIP, bool FREE>
 // Launch shape (chosen by the autotuner): any multiple of 64 up to 1024 threads per workgroup; with xcd != 0
 // workgroup b runs logical tile (b & 7) * chunk + (b >> 3), so consecutive tiles share an XCD and its L2.
 // The sweep number is *sweep_base + k: graphs replay fixed arguments while the base advances on the device.
@@ -377,7 +388,7 @@ __global__ __launch_bounds__(1024) void k_sweep(const LatDev *__restrict__ lats,
     const int t = tw + 64 * e;
     Site s;
     uint32_t self = 0;
-    if (t >= L.nc || !gather<MASK, SURF>(L, colour, t, X, Y, Z, PT, SX, SY, SZ, s, self)) continue;
+    if (t >= L.nc || !gather<MASK, SURF, FREE>(L, colour, t, X, Y, Z, PT, SX, SY, SZ, s, self)) continue;
     float u_ang, rot, u;
     if constexpr (SPT == 2) {
       const uint32_t w = R.v[2 * e];
@@ -819,6 +830,7 @@ struct Ctx {
   int nlat = 0;
   int pot = 0;
   bool mask = false, surf = false, field = false, flip = false;
+  bool free = false;  // some lattice has a free (non-periodic) axis
   bool xq = false;  // radial or pear observables are measured
   std::vector<LatHost> lh;
   std::vector<LatDev> ld;
@@ -854,9 +866,9 @@ struct Ctx {
 using SweepFn = void (*)(dim3, dim3, hipStream_t, const Ctx &, int, unsigned);
 using MeasFn = void (*)(dim3, dim3, hipStream_t, const Ctx &);
 
-template <int POT, bool MASK, bool SURF, bool FIELD, bool FLIP>
+template <int POT, bool MASK, bool SURF, bool FIELD, bool FLIP, bool FREE>
 void launch_sweep(dim3 g, dim3 b, hipStream_t st, const Ctx &c, int colour, unsigned k) {
-  hipLaunchKernelGGL((k_sweep<POT, MASK, SURF, FIELD, FLIP>), g, b, 0, st, c.lat_d, c.ctrl_d, c.cnt, c.blk_d, c.nblk,
+  hipLaunchKernelGGL((k_sweep<POT, MASK, SURF, FIELD, FLIP, FREE>), g, b, 0, st, c.lat_d, c.ctrl_d, c.cnt, c.blk_d, c.nblk,
                      c.chunk, c.xcd, colour, c.sweep_base, k, c.X, c.Y, c.Z, c.PT, c.SX, c.SY, c.SZ, c.surf_d);
 }
 template <int POT, bool MASK, bool SURF, bool FIELD>
@@ -869,25 +881,27 @@ void launch_measure(dim3 g, dim3 b, hipStream_t st, const Ctx &c) {
                        c.nmblk, c.X, c.Y, c.Z, c.PT, c.SX, c.SY, c.SZ, c.surf_d, c.partials);
 }
 
-template <int POT, bool MASK, bool SURF, bool FIELD>
+template <int POT, bool MASK, bool SURF, bool FIELD, bool FREE>
 SweepFn pick_sweep_flip(bool flip) {
-  return flip ? launch_sweep<POT, MASK, SURF, FIELD, true> : launch_sweep<POT, MASK, SURF, FIELD, false>;
+  return flip ? launch_sweep<POT, MASK, SURF, FIELD, true, FREE> : launch_sweep<POT, MASK, SURF, FIELD, false, FREE>;
 }
-template <int POT, bool MASK, bool SURF>
+template <int POT, bool MASK, bool SURF, bool FREE>
 SweepFn pick_sweep_field(bool field, bool flip) {
-  return field ? pick_sweep_flip<POT, MASK, SURF, true>(flip) : pick_sweep_flip<POT, MASK, SURF, false>(flip);
+  return field ? pick_sweep_flip<POT, MASK, SURF, true, FREE>(flip)
+               : pick_sweep_flip<POT, MASK, SURF, false, FREE>(flip);
 }
 template <int POT>
-SweepFn pick_sweep_geom(bool mask, bool surf, bool field, bool flip) {
-  if (surf) return pick_sweep_field<POT, true, true>(field, flip);
-  if (mask) return pick_sweep_field<POT, true, false>(field, flip);
-  return pick_sweep_field<POT, false, false>(field, flip);
+SweepFn pick_sweep_geom(bool mask, bool surf, bool free, bool field, bool flip) {
+  if (surf) return pick_sweep_field<POT, true, true, true>(field, flip);
+  if (mask) return pick_sweep_field<POT, true, false, true>(field, flip);
+  if (free) return pick_sweep_field<POT, false, false, true>(field, flip);
+  return pick_sweep_field<POT, false, false, false>(field, flip);
 }
 SweepFn pick_sweep(const Ctx &c) {
   switch (c.pot) {
-    case MCL_POT_GHRL: return pick_sweep_geom<MCL_POT_GHRL>(c.mask, c.surf, c.field, c.flip);
-    case MCL_POT_PEAR: return pick_sweep_geom<MCL_POT_PEAR>(c.mask, c.surf, c.field, c.flip);
-    default: return pick_sweep_geom<MCL_POT_LL>(c.mask, c.surf, c.field, c.flip);
+    case MCL_POT_GHRL: return pick_sweep_geom<MCL_POT_GHRL>(c.mask, c.surf, c.free, c.field, c.flip);
+    case MCL_POT_PEAR: return pick_sweep_geom<MCL_POT_PEAR>(c.mask, c.surf, c.free, c.field, c.flip);
+    default: return pick_sweep_geom<MCL_POT_LL>(c.mask, c.surf, c.free, c.field, c.flip);
   }
 }
 
@@ -1086,6 +1100,7 @@ int create_ctx(const mcl_lattice_desc *lat, int nlat, Ctx *c) {
     L.cy = (float)(d.ny / 2);
     L.cz = (float)(d.nz / 2);
     if (M.elecA != 0.0f || M.elecP != 0.0f) c->field = true;
+    if (!d.periodic[0] || !d.periodic[1] || !d.periodic[2]) c->free = true;
     if (d.radial != 0 || c->pot == MCL_POT_PEAR) c->xq = true;
     c->mask |= h.mask;
     c->surf |= h.surf;
