@@ -205,10 +205,9 @@ struct Site {
   int i, j, k;
 };
 
-// Site energy: bonds + anchoring + field (Bulk/Slab/Sphere/Custom_site_energy_GPU, geometry.cu:63-120).
-template <int POT, bool SURF, bool FIELD>
-__device__ __forceinline__ float site_energy(const LatDev &L, const Site &s, V3 n, const SurfDev *__restrict__ surf,
-                                             float elecE) {
+// Bond part of the site energy: the bonds to the existing neighbours (LL scaled by -1.5 A).
+template <int POT>
+__device__ __forceinline__ float bond_sum(const LatDev &L, const Site &s, V3 n) {
   float e = 0.0f;
   e += (s.ex & 1u) ? bond<POT, 0>(L, n, s.nb[0]) : 0.0f;
   e += (s.ex & 2u) ? bond<POT, 1>(L, n, s.nb[1]) : 0.0f;
@@ -217,13 +216,30 @@ __device__ __forceinline__ float site_energy(const LatDev &L, const Site &s, V3 
   e += (s.ex & 16u) ? bond<POT, 4>(L, n, s.nb[4]) : 0.0f;
   e += (s.ex & 32u) ? bond<POT, 5>(L, n, s.nb[5]) : 0.0f;
   if constexpr (POT == MCL_POT_LL) e *= -1.5f * L.A;
+  return e;
+}
+
+// One-site part of the site energy: anchoring on surface sites and the field.
+template <bool SURF, bool FIELD>
+__device__ __forceinline__ float site_terms(const LatDev &L, int ptv, V3 sv, V3 n, const SurfDev *__restrict__ surf,
+                                            float elecE) {
+  float e = 0.0f;
   if constexpr (SURF) {
-    if (s.ptv > 1) e += surface_energy(L, surf[L.surf0 + s.ptv - 2], n, s.sv);
+    if (ptv > 1) e += surface_energy(L, surf[L.surf0 + ptv - 2], n, sv);
   }
   if constexpr (FIELD) {
     const float nE = elecE * (n.x * L.ex + n.y * L.ey + n.z * L.ez);
     e += -L.elecP * nE - L.elecA * nE * nE;
   }
+  return e;
+}
+
+// Site energy: bonds + anchoring + field (Bulk/Slab/Sphere/Custom_site_energy_GPU, geometry.cu:63-120).
+template <int POT, bool SURF, bool FIELD>
+__device__ __forceinline__ float site_energy(const LatDev &L, const Site &s, V3 n, const SurfDev *__restrict__ surf,
+                                             float elecE) {
+  float e = bond_sum<POT>(L, s, n);
+  if constexpr (SURF || FIELD) e += site_terms<SURF, FIELD>(L, s.ptv, s.sv, n, surf, elecE);
   return e;
 }
 
@@ -297,6 +313,37 @@ __device__ __forceinline__ bool gather(const LatDev &L, int colour, int t, const
     }
   }
   s.ex = ex;
+  s.sv = V3{0.0f, 0.0f, 0.0f};
+  if constexpr (SURF) {
+    if (s.ptv > 1) s.sv = V3{ldx(SX, self), ldx(SY, self), ldx(SZ, self)};
+  }
+  return true;
+}
+
+// The site alone (no neighbours): decodes position t, loads its point type, director and anchoring vector.
+template <bool MASK, bool SURF>
+__device__ __forceinline__ bool gather_self(const LatDev &L, int colour, int t, const float *__restrict__ X,
+                                            const float *__restrict__ Y, const float *__restrict__ Z,
+                                            const uint8_t *__restrict__ PT, const float *__restrict__ SX,
+                                            const float *__restrict__ SY, const float *__restrict__ SZ, Site &s) {
+  const uint32_t row = fdiv((uint32_t)t, L.mxh, L.sxh);
+  const int m = t - (int)row * L.nxh;
+  const int k = (int)fdiv(row, L.my, L.sy);
+  const int j = (int)row - k * L.ny;
+  if (m < 0 || m >= L.nxh || j < 0 || j >= L.ny || k >= L.nz) return false;
+  const int i = 2 * m + ((j + k + colour) & 1);
+  if (i >= L.nx) return false;
+  s.i = i;
+  s.j = j;
+  s.k = k;
+  const uint32_t self = L.off + (uint32_t)(colour * L.nc) + (uint32_t)t;
+  s.ptv = 1;
+  if constexpr (MASK) {
+    s.ptv = ldx(PT, self);
+    if (s.ptv == 0) return false;
+  }
+  s.n = V3{ldx(X, self), ldx(Y, self), ldx(Z, self)};
+  s.ex = 0u;
   s.sv = V3{0.0f, 0.0f, 0.0f};
   if constexpr (SURF) {
     if (s.ptv > 1) s.sv = V3{ldx(SX, self), ldx(SY, self), ldx(SZ, self)};
@@ -470,7 +517,11 @@ __device__ __forceinline__ double wave_sum(double v) {
 // The basic quantities (energy, Q sums, count) are 0..6 and 21; XQ adds the radial, polar and pear ones.
 __host__ __device__ constexpr bool basic_q(int a) { return a <= 6 || a == 21; }
 
-template <int POT, bool MASK, bool SURF, bool FIELD, bool XQ>
+// The energy of a lattice counts every bond twice (once from each end, as the reference's per-site sum does) plus
+// the one-site terms of every site. Every bond has exactly one colour-0 end, so the colour-0 workgroups add twice
+// their sites' bond sums and the colour-1 workgroups only the one-site terms, which need no neighbours (the pear
+// observables, which do, gather them on both colours).
+template <int POT, bool MASK, bool SURF, bool FIELD, bool XQ, bool FREE>
 __global__ __launch_bounds__(kBlock) void k_measure(const LatDev *__restrict__ lats, const CtrlDev *__restrict__ ctrl,
                                                     const int4 *__restrict__ mblk, int nmblk,
                                                     const float *__restrict__ X, const float *__restrict__ Y,
@@ -488,13 +539,22 @@ __global__ __launch_bounds__(kBlock) void k_measure(const LatDev *__restrict__ l
   double q[kNQ];
 #pragma unroll
   for (int a = 0; a < kNQ; a++) q[a] = 0.0;
+  const bool bonds = colour == 0;
+  constexpr bool kAllNeighbours = XQ && POT == MCL_POT_PEAR;
   for (int it = 0; it < kMeasPerThread; it++) {
     const int t = md.z + it * kBlock + (int)threadIdx.x;
     Site s;
     uint32_t self;
-    if (t >= md.w || !gather<MASK, SURF>(L, colour, t, X, Y, Z, PT, SX, SY, SZ, s, self)) continue;
+    if (t >= md.w) continue;
+    if (bonds || kAllNeighbours) {
+      if (!gather<MASK, SURF, FREE>(L, colour, t, X, Y, Z, PT, SX, SY, SZ, s, self)) continue;
+    } else {
+      if (!gather_self<MASK, SURF>(L, colour, t, X, Y, Z, PT, SX, SY, SZ, s)) continue;
+    }
     const V3 n = s.n;
-    q[0] += (double)site_energy<POT, SURF, FIELD>(L, s, n, surf, elecE);
+    double e = bonds ? 2.0 * (double)bond_sum<POT>(L, s, n) : 0.0;
+    if constexpr (SURF || FIELD) e += (double)site_terms<SURF, FIELD>(L, s.ptv, s.sv, n, surf, elecE);
+    q[0] += e;
     const double x = n.x, y = n.y, z = n.z;
     q[1] += x * x;
     q[2] += x * y;
@@ -871,13 +931,13 @@ void launch_sweep(dim3 g, dim3 b, hipStream_t st, const Ctx &c, int colour, unsi
   hipLaunchKernelGGL((k_sweep<POT, MASK, SURF, FIELD, FLIP, FREE>), g, b, 0, st, c.lat_d, c.ctrl_d, c.cnt, c.blk_d, c.nblk,
                      c.chunk, c.xcd, colour, c.sweep_base, k, c.X, c.Y, c.Z, c.PT, c.SX, c.SY, c.SZ, c.surf_d);
 }
-template <int POT, bool MASK, bool SURF, bool FIELD>
+template <int POT, bool MASK, bool SURF, bool FIELD, bool FREE>
 void launch_measure(dim3 g, dim3 b, hipStream_t st, const Ctx &c) {
   if (c.xq)
-    hipLaunchKernelGGL((k_measure<POT, MASK, SURF, FIELD, true>), g, b, 0, st, c.lat_d, c.ctrl_d, c.mblk_d,
+    hipLaunchKernelGGL((k_measure<POT, MASK, SURF, FIELD, true, FREE>), g, b, 0, st, c.lat_d, c.ctrl_d, c.mblk_d,
                        c.nmblk, c.X, c.Y, c.Z, c.PT, c.SX, c.SY, c.SZ, c.surf_d, c.partials);
   else
-    hipLaunchKernelGGL((k_measure<POT, MASK, SURF, FIELD, false>), g, b, 0, st, c.lat_d, c.ctrl_d, c.mblk_d,
+    hipLaunchKernelGGL((k_measure<POT, MASK, SURF, FIELD, false, FREE>), g, b, 0, st, c.lat_d, c.ctrl_d, c.mblk_d,
                        c.nmblk, c.X, c.Y, c.Z, c.PT, c.SX, c.SY, c.SZ, c.surf_d, c.partials);
 }
 
@@ -905,21 +965,22 @@ SweepFn pick_sweep(const Ctx &c) {
   }
 }
 
-template <int POT, bool MASK, bool SURF>
+template <int POT, bool MASK, bool SURF, bool FREE>
 MeasFn pick_meas_field(bool field) {
-  return field ? launch_measure<POT, MASK, SURF, true> : launch_measure<POT, MASK, SURF, false>;
+  return field ? launch_measure<POT, MASK, SURF, true, FREE> : launch_measure<POT, MASK, SURF, false, FREE>;
 }
 template <int POT>
-MeasFn pick_meas_geom(bool mask, bool surf, bool field) {
-  if (surf) return pick_meas_field<POT, true, true>(field);
-  if (mask) return pick_meas_field<POT, true, false>(field);
-  return pick_meas_field<POT, false, false>(field);
+MeasFn pick_meas_geom(bool mask, bool surf, bool free, bool field) {
+  if (surf) return pick_meas_field<POT, true, true, true>(field);
+  if (mask) return pick_meas_field<POT, true, false, true>(field);
+  if (free) return pick_meas_field<POT, false, false, true>(field);
+  return pick_meas_field<POT, false, false, false>(field);
 }
 MeasFn pick_meas(const Ctx &c) {
   switch (c.pot) {
-    case MCL_POT_GHRL: return pick_meas_geom<MCL_POT_GHRL>(c.mask, c.surf, c.field);
-    case MCL_POT_PEAR: return pick_meas_geom<MCL_POT_PEAR>(c.mask, c.surf, c.field);
-    default: return pick_meas_geom<MCL_POT_LL>(c.mask, c.surf, c.field);
+    case MCL_POT_GHRL: return pick_meas_geom<MCL_POT_GHRL>(c.mask, c.surf, c.free, c.field);
+    case MCL_POT_PEAR: return pick_meas_geom<MCL_POT_PEAR>(c.mask, c.surf, c.free, c.field);
+    default: return pick_meas_geom<MCL_POT_LL>(c.mask, c.surf, c.free, c.field);
   }
 }
 
