@@ -436,3 +436,18 @@ def test_electric_run_aligns_with_a_strong_field(orc):
     assert rows[0, 3] - rows[1, 3] > 3.0
     lam, v, _ = orc.eig_sym3(orc.q_tensor(ni, pt))
     assert abs(v[0]) > 0.99
+
+
+def test_thermal_loop_reproduces_the_c1_fixture_prefix(orc):
+    """The oracle's thermal loop is deterministic for a fixed seed and thread count: the first temperatures of the
+    committed 16^3 fixture (replica 1, one thread) come out bit for bit, so a change to the loop code that alters the
+    trajectory is caught here."""
+    import os
+    ref = np.load(os.path.join(os.path.dirname(__file__), "golden", "c1_oracle.npz"))["rows"][0]
+    N = 16
+    pt, _ = orc.geometry("bulk", N, N, N)
+    p = orc.params(N, N, N, periodic=(1, 1, 1), potential="ll", A=1.0)
+    ni = orc.ic_random(pt, 1)
+    f32 = lambda x: float(np.float32(x))
+    rows = orc.thermal_run(p, ni, pt, None, f32(1.2), f32(1.1), f32(-0.02), 500, 200, 1, 1)
+    assert np.array_equal(rows[:5], ref[:5])
