@@ -40,6 +40,8 @@ namespace {
 
 constexpr int kBlock = 256;          // threads per workgroup (4 wave64)
 constexpr int kWaves = kBlock / 64;
+constexpr int kFinBlock = 1024;  // finalize: one partial set per thread for up to 1024 workgroups of a lattice
+constexpr int kFinWaves = kFinBlock / 64;
 constexpr int kMeasPerThread = 8;    // sites per thread in a measurement workgroup
 constexpr int kNQ = 24;              // FP64 quantities per measurement partial
 constexpr uint32_t kTagSweep = 0x4D435357u;  // counter word 3 of the sweep draws
@@ -351,6 +353,13 @@ __device__ __forceinline__ bool gather_self(const LatDev &L, int colour, int t, 
   return true;
 }
 
+// Sum of a double over the 64 lanes of a wave (every lane gets the total).
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
 // ang_var rule of evolve.cu:430-436. The reference compares the double quotient acc / nvalid with 0.5; for
 // integers below 2^31 that is the same as comparing 2 acc with nvalid (the quotient is never within rounding of
 // 0.5 unless it equals 0.5), which avoids an FP64 division. The updates keep the reference's float * double
@@ -372,21 +381,93 @@ __host__ __device__ __forceinline__ float ang_var_rule(float av, unsigned acc, i
 // Sweep pass over one colour of every lattice. blk[L] = (lattice, first colour-block position) of logical
 // workgroup L; consecutive logical workgroups are placed on the same XCD (workgroups are dispatched to the 8 XCDs
 // round robin), so the rows a workgroup shares with its neighbours in z and y mostly hit the same L2.
-template <int POT, bool MASK, bool SURF, bool FIELD, bool FLIP, bool FREE>
+__device__ __forceinline__ double eig_sym3_max(const double a[6], double v[3]);
+
+// Compact layout of the measurement partials written by the sweep itself: 0 E, 1..5 sum n_a n_b (xx xy xz yy yz),
+// 6 count, 7 sum n_z^2.
+constexpr int kNQS = 8;
+
+// Finalize of one lattice's compact partials [rg.x, rg.y) (column stride `stride`) by a whole workgroup of any
+// multiple of 64 threads: fixed-order sum, observables, and the sample added to the moments. The quantities of the
+// radial and pear observables are zero in this layout.
+__device__ void finalize_compact(int r, const double *__restrict__ part, int stride, int2 rg,
+                                 MomDev *__restrict__ mom, ObsDev *__restrict__ obs) {
+  __shared__ double fred[16][kNQS];
+  double v[kNQS];
+#pragma unroll
+  for (int a = 0; a < kNQS; a++) v[a] = 0.0;
+  for (int p = rg.x + (int)threadIdx.x; p < rg.y; p += (int)blockDim.x) {
+#pragma unroll
+    for (int a = 0; a < kNQS; a++) v[a] += part[(size_t)a * stride + p];
+  }
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int a = 0; a < kNQS; a++) {
+    const double x = wave_sum(v[a]);
+    if (lane == 0) fred[w][a] = x;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  double t[kNQS];
+  const int nw = (int)(blockDim.x >> 6);
+  for (int a = 0; a < kNQS; a++) {
+    double x = 0.0;
+    for (int ww = 0; ww < nw; ww++) x += fred[ww][a];
+    t[a] = x;
+  }
+  const double N = t[6], inv = N > 0.0 ? 1.0 / N : 0.0;
+  ObsDev o = {};
+  o.nvalid = N;
+  o.E = t[0] * inv;
+  o.Q[0] = (3.0 * t[1] - N) * 0.5 * inv;
+  o.Q[1] = 3.0 * t[2] * 0.5 * inv;
+  o.Q[2] = 3.0 * t[3] * 0.5 * inv;
+  o.Q[3] = (3.0 * t[4] - N) * 0.5 * inv;
+  o.Q[4] = 3.0 * t[5] * 0.5 * inv;
+  o.Q[5] = (3.0 * t[7] - N) * 0.5 * inv;
+  double qm[6] = {o.Q[0], o.Q[1], o.Q[2], o.Q[3], o.Q[4], -o.Q[0] - o.Q[3]};
+  o.S = N > 0.0 ? eig_sym3_max(qm, o.dir) : 0.0;
+  if (N <= 0.0) { o.dir[0] = 0; o.dir[1] = 0; o.dir[2] = 1; }
+  obs[r] = o;
+  MomDev m = mom[r];
+  m.n += 1.0;
+  m.S += o.S;
+  m.S2 += o.S * o.S;
+  m.E += o.E;
+  m.E2 += o.E * o.E;
+  mom[r] = m;
+}
+
+template <int POT, bool MASK, bool SURF, bool FIELD, bool FLIP, bool FREE, bool MEAS>
 // Launch shape (chosen by the autotuner): any multiple of 64 up to 1024 threads per workgroup; with xcd != 0
 // workgroup b runs logical tile (b & 7) * chunk + (b >> 3), so consecutive tiles share an XCD and its L2.
 // The sweep number is *sweep_base + k: graphs replay fixed arguments while the base advances on the device.
-__global__ __launch_bounds__(1024) void k_sweep(const LatDev *__restrict__ lats, CtrlDev *__restrict__ ctrl,
+// MEAS (colour-1 pass of a measured sweep, unmasked lattices with at least two sites along x): the pass also
+// measures the lattice it leaves behind. Every bond has exactly one colour-1 end and every colour-1 site's final
+// energy is known from its trial, so the lattice energy is the sum over colour-1 sites of twice their final site
+// energy minus their one-site terms, plus the one-site terms of colour 0. Each colour-0 site is counted (Q tensor,
+// one-site terms, count) by the colour-1 site on its +x side, or, at the last column of a free x axis, by the one on
+// its -x side. The workgroup writes kNQS FP64 partials to spart[q * nblk + workgroup].
+__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_sweep(
+    const LatDev *__restrict__ lats, CtrlDev *__restrict__ ctrl,
                                                 unsigned int *__restrict__ cnt, const int2 *__restrict__ blk,
                                                 int nblk, int chunk, int xcd, int colour,
                                                 const unsigned long long *__restrict__ sweep_base, unsigned k,
                                                   float *__restrict__ X, float *__restrict__ Y,
                                                   float *__restrict__ Z, const uint8_t *__restrict__ PT,
                                                   const float *__restrict__ SX, const float *__restrict__ SY,
-                                                  const float *__restrict__ SZ, const SurfDev *__restrict__ surf) {
+                                                  const float *__restrict__ SZ, const SurfDev *__restrict__ surf,
+                                                  double *__restrict__ spart, int fin, int nlat,
+                                                  const int2 *__restrict__ srange, MomDev *__restrict__ mom,
+                                                  ObsDev *__restrict__ obs) {
+  static_assert(!(MEAS && MASK), "the fused measurement needs unmasked lattices");
   const int b = blockIdx.x;
   const int Lb = xcd ? (b & 7) * chunk + (b >> 3) : b;
-  if (Lb >= nblk) return;
+  if (Lb >= nblk) {
+    // the workgroups past the work list finalize the measurement of the previous sweep (fin = 1), one per lattice
+    if (fin && Lb < nblk + nlat) finalize_compact(Lb - nblk, spart, nblk, srange[Lb - nblk], mom, obs);
+    return;
+  }
   const int2 bd = blk[Lb];
   const int r = bd.x;
   const LatDev &L = lats[r];
@@ -430,6 +511,19 @@ __global__ __launch_bounds__(1024) void k_sweep(const LatDev *__restrict__ lats,
   const mcl::Philox4 R = mcl::philox4x32_10(ctr0, (uint32_t)sweep, (uint32_t)(sweep >> 32),
                                             kTagSweep + (uint32_t)colour, L.key0, L.key1);
   unsigned nacc = 0;
+  double qm[kNQS];
+#pragma unroll
+  for (int a = 0; a < kNQS; a++) qm[a] = 0.0;
+  auto add_q = [&](V3 v) {
+    const double x = v.x, y = v.y, z = v.z;
+    qm[1] += x * x;
+    qm[2] += x * y;
+    qm[3] += x * z;
+    qm[4] += y * y;
+    qm[5] += y * z;
+    qm[7] += z * z;
+    qm[6] += 1.0;
+  };
 #pragma unroll
   for (int e = 0; e < SPT; e++) {
     const int t = tw + 64 * e;
@@ -470,15 +564,16 @@ __global__ __launch_bounds__(1024) void k_sweep(const LatDev *__restrict__ lats,
     const bool accepted = T > 0.0f ? (u <= __expf(-(E_new - E_old) * invT)) : (E_new <= E_old);
     nacc += accepted ? 1u : 0u;
     V3 nc = accepted ? nn : n;
+    float E_fin = accepted ? E_new : E_old;
     bool changed = accepted;
     if constexpr (FLIP) {
       // separate n -> -n trial, not counted in the acceptance (evolve.cu:225-239)
-      const float E_cur = accepted ? E_new : E_old;
       const V3 nf{-nc.x, -nc.y, -nc.z};
       const float E_f = site_energy<POT, SURF, FIELD>(L, s, nf, surf, elecE);
       const float uf = mcl::u01_open(R.v[3]);
-      if (T > 0.0f ? (uf <= __expf(-(E_f - E_cur) * invT)) : (E_f <= E_cur)) {
+      if (T > 0.0f ? (uf <= __expf(-(E_f - E_fin) * invT)) : (E_f <= E_fin)) {
         nc = nf;
+        E_fin = E_f;
         changed = true;
       }
     }
@@ -486,6 +581,23 @@ __global__ __launch_bounds__(1024) void k_sweep(const LatDev *__restrict__ lats,
       stx(X, self, nc.x);
       stx(Y, self, nc.y);
       stx(Z, self, nc.z);
+    }
+    if constexpr (MEAS) {
+      double e2 = 2.0 * (double)E_fin;
+      add_q(nc);
+      if constexpr (FIELD) e2 -= (double)site_terms<false, true>(L, 1, V3{0.0f, 0.0f, 0.0f}, nc, surf, elecE);
+      // the colour-0 sites this site counts: its -x neighbour, and its +x neighbour at the end of a free x axis
+      const bool own_m = FREE ? (s.ex & 2u) != 0 : true;
+      const bool own_p = FREE && !(L.flags & F_PX) && s.i + 1 == L.nx - 1;
+      if (own_m) {
+        add_q(s.nb[1]);
+        if constexpr (FIELD) e2 += (double)site_terms<false, true>(L, 1, V3{0.0f, 0.0f, 0.0f}, s.nb[1], surf, elecE);
+      }
+      if (own_p) {
+        add_q(s.nb[0]);
+        if constexpr (FIELD) e2 += (double)site_terms<false, true>(L, 1, V3{0.0f, 0.0f, 0.0f}, s.nb[0], surf, elecE);
+      }
+      qm[0] += e2;
     }
   }
 
@@ -502,17 +614,27 @@ __global__ __launch_bounds__(1024) void k_sweep(const LatDev *__restrict__ lats,
     for (int w = 0; w < nw; w++) tot += wsum[w];
     if (tot) atomicAdd(&cnt_r[(sweep % 3) * kCntSweep + (lblk & (kCnt - 1)) * kCntStride], tot);
   }
+  if constexpr (MEAS) {
+    __shared__ double mred[16][kNQS];
+#pragma unroll
+    for (int a = 0; a < kNQS; a++) {
+      const double v = wave_sum(qm[a]);
+      if (lane == 0) mred[threadIdx.x >> 6][a] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < kNQS) {
+      double v = 0.0;
+      const int nw = (int)(blockDim.x >> 6);
+      for (int w = 0; w < nw; w++) v += mred[w][threadIdx.x];
+      spart[(size_t)threadIdx.x * nblk + Lb] = v;
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------------------------------------------
 // Measurement: per-workgroup FP64 partial sums over kBlock * kMeasPerThread sites of one colour of one lattice.
 // Quantities: 0 E, 1..5 sum n_a n_b (xx xy xz yy yz), 6 nvalid, 7..9 radial sphere (sum Sr, sum c, count),
 // 10..12 radial cylinder, 13..15 sum n, 16..18 splay vector, 19 bonds, 20 sum div n, 21 sum n_z^2.
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
-}
 
 // The basic quantities (energy, Q sums, count) are 0..6 and 21; XQ adds the radial, polar and pear ones.
 __host__ __device__ constexpr bool basic_q(int a) { return a <= 6 || a == 21; }
@@ -645,7 +767,7 @@ __global__ __launch_bounds__(kBlock) void k_measure(const LatDev *__restrict__ l
 // Largest eigenvalue and eigenvector of a symmetric 3x3 matrix in FP64: trigonometric solution of the
 // characteristic cubic (O. K. Smith, "Eigenvalues of a symmetric 3x3 matrix", CACM 4(4):168, 1961); the
 // eigenvector is the largest cross product of two rows of A - lambda I.
-__device__ double eig_sym3_max(const double a[6], double v[3]) {
+__device__ __forceinline__ double eig_sym3_max(const double a[6], double v[3]) {
   // a = (xx, xy, xz, yy, yz, zz)
   const double a00 = a[0], a01 = a[1], a02 = a[2], a11 = a[3], a12 = a[4], a22 = a[5];
   const double p1 = a01 * a01 + a02 * a02 + a12 * a12;
@@ -688,24 +810,37 @@ __device__ double eig_sym3_max(const double a[6], double v[3]) {
 
 // Sums the partials of each lattice in a fixed order (one workgroup per lattice), computes the observables and,
 // when accumulate != 0, adds the sample to the moments.
-__global__ __launch_bounds__(kBlock) void k_finalize(const LatDev *__restrict__ lats,
+// Partials come either from k_measure (ranges == nullptr: the lattice's range [mp0, mp1), quantities 0..kNQ-1) or
+// from the measuring sweep pass (ranges[r]: the lattice's workgroups of the sweep work list, compact layout of kNQS
+// basic quantities, xq = 0).
+__global__ __launch_bounds__(kFinBlock) void k_finalize(const LatDev *__restrict__ lats,
                                                      const double *__restrict__ partials, int nmblk, int xq,
                                                      MomDev *__restrict__ mom, ObsDev *__restrict__ obs,
-                                                     int accumulate) {
+                                                     int accumulate, const int2 *__restrict__ ranges) {
   const int r = blockIdx.x;
   const LatDev &L = lats[r];
-  __shared__ double red[kWaves][kNQ];
+  __shared__ double red[kFinWaves][kNQ];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   double v[kNQ];
 #pragma unroll
   for (int a = 0; a < kNQ; a++) v[a] = 0.0;
-  for (int p = L.mp0 + (int)threadIdx.x; p < L.mp1; p += kBlock) {
+  if (ranges) {
+    const int2 rg = ranges[r];
+    for (int p = rg.x + (int)threadIdx.x; p < rg.y; p += kFinBlock) {
 #pragma unroll
-    for (int a = 0; a < kNQ; a++)
-      if (xq || basic_q(a)) v[a] += partials[(size_t)a * nmblk + p];
+      for (int a = 0; a < 7; a++) v[a] += partials[(size_t)a * nmblk + p];
+      v[21] += partials[(size_t)7 * nmblk + p];
+    }
+  } else {
+    for (int p = L.mp0 + (int)threadIdx.x; p < L.mp1; p += kFinBlock) {
+#pragma unroll
+      for (int a = 0; a < kNQ; a++)
+        if (xq || basic_q(a)) v[a] += partials[(size_t)a * nmblk + p];
+    }
   }
 #pragma unroll
   for (int a = 0; a < kNQ; a++) {
+    if (!xq && !basic_q(a)) continue;  // quantities never written are left out of the reduction
     const double x = wave_sum(v[a]);
     if (lane == 0) red[w][a] = x;
   }
@@ -714,7 +849,8 @@ __global__ __launch_bounds__(kBlock) void k_finalize(const LatDev *__restrict__ 
   double tot[kNQ];
   for (int a = 0; a < kNQ; a++) {
     double x = 0.0;
-    for (int ww = 0; ww < kWaves; ww++) x += red[ww][a];
+    if (xq || basic_q(a))
+      for (int ww = 0; ww < kFinWaves; ww++) x += red[ww][a];
     tot[a] = x;
   }
   const double N = tot[6];
@@ -912,6 +1048,11 @@ struct Ctx {
   int4 *mblk_d = nullptr;
   int nmblk = 0;
   double *partials = nullptr;
+  // measurement folded into the colour-1 sweep pass (unmasked lattices without the radial or pear observables)
+  bool fused = false;
+  double *spart = nullptr;  // kNQS partials per workgroup of the sweep work list
+  size_t spart_cap = 0;     // workgroups spart holds
+  int2 *sr_d = nullptr;     // per lattice: its workgroups [first, last) in the sweep work list
   MomDev *mom_d = nullptr;
   ObsDev *obs_d = nullptr;
   float *scratch_f = nullptr;  // Nt floats of the largest lattice (local order output)
@@ -923,14 +1064,27 @@ struct Ctx {
   std::vector<std::pair<size_t, int>> ev_runs;  // (first event of the pair, kernel launches in the run)
 };
 
-using SweepFn = void (*)(dim3, dim3, hipStream_t, const Ctx &, int, unsigned);
+using SweepFn = void (*)(dim3, dim3, hipStream_t, const Ctx &, int, unsigned, bool, int);
 using MeasFn = void (*)(dim3, dim3, hipStream_t, const Ctx &);
 
 template <int POT, bool MASK, bool SURF, bool FIELD, bool FLIP, bool FREE>
-void launch_sweep(dim3 g, dim3 b, hipStream_t st, const Ctx &c, int colour, unsigned k) {
-  hipLaunchKernelGGL((k_sweep<POT, MASK, SURF, FIELD, FLIP, FREE>), g, b, 0, st, c.lat_d, c.ctrl_d, c.cnt, c.blk_d, c.nblk,
-                     c.chunk, c.xcd, colour, c.sweep_base, k, c.X, c.Y, c.Z, c.PT, c.SX, c.SY, c.SZ, c.surf_d);
+void launch_sweep(dim3 g, dim3 b, hipStream_t st, const Ctx &c, int colour, unsigned k, bool meas, int fin) {
+  if constexpr (!MASK) {
+    if (meas) {
+      hipLaunchKernelGGL((k_sweep<POT, MASK, SURF, FIELD, FLIP, FREE, true>), g, b, 0, st, c.lat_d, c.ctrl_d, c.cnt,
+                         c.blk_d, c.nblk, c.chunk, c.xcd, colour, c.sweep_base, k, c.X, c.Y, c.Z, c.PT, c.SX, c.SY,
+                         c.SZ, c.surf_d, c.spart, fin, c.nlat, c.sr_d, c.mom_d, c.obs_d);
+      return;
+    }
+  }
+  hipLaunchKernelGGL((k_sweep<POT, MASK, SURF, FIELD, FLIP, FREE, false>), g, b, 0, st, c.lat_d, c.ctrl_d, c.cnt,
+                     c.blk_d, c.nblk, c.chunk, c.xcd, colour, c.sweep_base, k, c.X, c.Y, c.Z, c.PT, c.SX, c.SY, c.SZ,
+                     c.surf_d, c.spart, fin, c.nlat, c.sr_d, c.mom_d, c.obs_d);
 }
+
+// Grid of a sweep launch: the work list, plus one workgroup per lattice for the deferred finalize of a measurement
+// folded into the previous sweep.
+dim3 sweep_grid(const Ctx *c) { return dim3(c->xcd ? 8 * c->chunk : c->nblk + (c->fused ? c->nlat : 0)); }
 template <int POT, bool MASK, bool SURF, bool FIELD, bool FREE>
 void launch_measure(dim3 g, dim3 b, hipStream_t st, const Ctx &c) {
   if (c.xq)
@@ -1028,7 +1182,8 @@ void destroy_ctx(Ctx *c) {
   for (auto e : c->ev) (void)hipEventDestroy(e);
   for (auto &g : c->graphs) (void)hipGraphExecDestroy(g.second);
   void *bufs[] = {c->X, c->Y, c->Z, c->PT, c->SX, c->SY, c->SZ, c->surf_d, c->lat_d, c->ctrl_d,
-                  c->blk_d, c->cnt, c->sweep_base, c->mblk_d, c->partials, c->mom_d, c->obs_d, c->scratch_f};
+                  c->blk_d, c->cnt, c->sweep_base, c->mblk_d, c->partials, c->mom_d, c->obs_d, c->scratch_f,
+                  c->spart, c->sr_d};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -1049,9 +1204,12 @@ int dev_alloc(T **p, size_t n) {
 int build_worklist(Ctx *c, int bs, int xcd) {
   // a workgroup of the sweep covers bs colour-block positions, or 2 bs when a thread updates a pair of sites
   const int span = c->flip ? bs : 2 * bs;
-  std::vector<int2> blk;
-  for (int r = 0; r < c->nlat; r++)
+  std::vector<int2> blk, sr(c->nlat);
+  for (int r = 0; r < c->nlat; r++) {
+    sr[r].x = (int)blk.size();
     for (int t0 = 0; t0 < c->ld[r].nc; t0 += span) blk.push_back(make_int2(r, t0));
+    sr[r].y = (int)blk.size();
+  }
   if (c->blk_d) {
     MCL_HIP(hipStreamSynchronize(c->stream));
     MCL_HIP(hipFree(c->blk_d));
@@ -1060,8 +1218,20 @@ int build_worklist(Ctx *c, int bs, int xcd) {
   int st;
   if ((st = dev_alloc(&c->blk_d, blk.size()))) return st;
   MCL_HIP(hipMemcpy(c->blk_d, blk.data(), sizeof(int2) * blk.size(), hipMemcpyHostToDevice));
+  if (c->fused) {
+    if (blk.size() > c->spart_cap) {
+      MCL_HIP(hipStreamSynchronize(c->stream));
+      if (c->spart) MCL_HIP(hipFree(c->spart));
+      c->spart = nullptr;
+      c->spart_cap = 0;
+      if ((st = dev_alloc(&c->spart, (size_t)kNQS * blk.size()))) return st;
+      c->spart_cap = blk.size();
+    }
+    if (!c->sr_d && (st = dev_alloc(&c->sr_d, (size_t)c->nlat))) return st;
+    MCL_HIP(hipMemcpy(c->sr_d, sr.data(), sizeof(int2) * sr.size(), hipMemcpyHostToDevice));
+  }
   c->nblk = (int)blk.size();
-  c->chunk = (c->nblk + 7) / 8;
+  c->chunk = (c->nblk + (c->fused ? c->nlat : 0) + 7) / 8;
   c->bs = bs;
   c->xcd = xcd;
   for (auto &g : c->graphs) (void)hipGraphExecDestroy(g.second);
@@ -1216,6 +1386,9 @@ int create_ctx(const mcl_lattice_desc *lat, int nlat, Ctx *c) {
   MCL_HIP(hipMemcpyAsync(c->lat_d, c->ld.data(), sizeof(LatDev) * nlat, hipMemcpyHostToDevice, c->stream));
   const unsigned long long zero_base = 0ull;
   MCL_HIP(hipMemcpyAsync(c->sweep_base, &zero_base, sizeof(zero_base), hipMemcpyHostToDevice, c->stream));
+  c->fused = !c->mask && !c->xq;
+  for (int r = 0; r < nlat; r++)
+    if (c->ld[r].nx < 2) c->fused = false;
   if ((st = build_worklist(c, 256, 1))) return st;
   MCL_HIP(hipMemcpyAsync(c->mblk_d, mblk.data(), sizeof(int4) * mblk.size(), hipMemcpyHostToDevice, c->stream));
   std::vector<CtrlDev> ctrl(nlat);
@@ -1296,8 +1469,8 @@ int create_ctx(const mcl_lattice_desc *lat, int nlat, Ctx *c) {
 int run_measure(Ctx *c, int accumulate) {
   const MeasFn mf = pick_meas(*c);
   mf(dim3(c->nmblk), dim3(kBlock), c->stream, *c);
-  hipLaunchKernelGGL(k_finalize, dim3(c->nlat), dim3(kBlock), 0, c->stream, c->lat_d, c->partials, c->nmblk,
-                     c->xq ? 1 : 0, c->mom_d, c->obs_d, accumulate);
+  hipLaunchKernelGGL(k_finalize, dim3(c->nlat), dim3(kFinBlock), 0, c->stream, c->lat_d, c->partials, c->nmblk,
+                     c->xq ? 1 : 0, c->mom_d, c->obs_d, accumulate, (const int2 *)nullptr);
   MCL_HIP(hipGetLastError());
   return MCL_OK;
 }
@@ -1365,17 +1538,44 @@ int mcl_create(const mcl_lattice_desc *lat, int32_t nlat, void **ctx_out) {
 void mcl_destroy(void *ctx) { destroy_ctx(as_ctx(ctx)); }
 
 // Queues sweeps [k0, k0 + n) relative to the device sweep base (no timing, no advance).
-static int enqueue_sweeps(Ctx *c, SweepFn sf, int k0, int n, int measure_every) {
-  const dim3 grid(c->xcd ? 8 * c->chunk : c->nblk), block(c->bs);
-  for (int s = 0; s < n; s++) {
-    sf(grid, block, c->stream, *c, 0, (unsigned)(k0 + s));
-    sf(grid, block, c->stream, *c, 1, (unsigned)(k0 + s));
-    if (measure_every > 0 && (s + 1) % measure_every == 0) {
-      const int st = run_measure(c, 1);
-      if (st) return st;
-    }
-  }
+// The finalize of a measurement taken by the colour-1 sweep pass.
+static int run_fused_finalize(Ctx *c) {
+  hipLaunchKernelGGL(k_finalize, dim3(c->nlat), dim3(kFinBlock), 0, c->stream, c->lat_d, c->spart, c->nblk, 0,
+                     c->mom_d, c->obs_d, 1, (const int2 *)c->sr_d);
+  MCL_HIP(hipGetLastError());
   return MCL_OK;
+}
+
+// One sweep (both colour passes) and, when measure is set, its measurement: folded into the colour-1 pass when the
+// context allows it (its finalize then runs in the extra workgroups of the next colour-0 pass, *pending, or in a
+// k_finalize launch at the end of the call), else the measurement kernels after the pass.
+static int sweep_once(Ctx *c, SweepFn sf, dim3 grid, dim3 block, unsigned k, bool measure, bool *pending) {
+  sf(grid, block, c->stream, *c, 0, k, false, *pending ? 1 : 0);
+  *pending = false;
+  sf(grid, block, c->stream, *c, 1, k, measure && c->fused, 0);
+  if (!measure) return MCL_OK;
+  if (c->fused) {
+    *pending = true;
+    return MCL_OK;
+  }
+  return run_measure(c, 1);
+}
+
+static int flush_finalize(Ctx *c, bool *pending) {
+  if (!*pending) return MCL_OK;
+  *pending = false;
+  return run_fused_finalize(c);
+}
+
+static int enqueue_sweeps(Ctx *c, SweepFn sf, int k0, int n, int measure_every) {
+  const dim3 grid = sweep_grid(c), block(c->bs);
+  bool pending = false;
+  for (int s = 0; s < n; s++) {
+    const int st = sweep_once(c, sf, grid, block, (unsigned)(k0 + s),
+                              measure_every > 0 && (s + 1) % measure_every == 0, &pending);
+    if (st) return st;
+  }
+  return flush_finalize(c, &pending);
 }
 
 // Graph of `len` sweeps (with the measurements of measure_every) followed by the advance of the sweep base.
@@ -1410,7 +1610,7 @@ int mcl_sweeps(void *ctx, int32_t nsweeps, int32_t measure_every) {
   if (nsweeps < 0 || measure_every < 0) return fail(MCL_ERR_ARG, "negative sweep count");
   if (nsweeps == 0) return MCL_OK;
   const SweepFn sf = pick_sweep(*c);
-  const dim3 grid(c->xcd ? 8 * c->chunk : c->nblk), block(c->bs);
+  const dim3 grid = sweep_grid(c), block(c->bs);
   int done = 0;
   if (c->graph_block > 0 && c->timing_stride == 0) {
     // whole graphs of len sweeps (a multiple of measure_every, so the measurements fall where the eager path
@@ -1431,6 +1631,7 @@ int mcl_sweeps(void *ctx, int32_t nsweeps, int32_t measure_every) {
     // cut at every measurement), so the markers do not sit between the timed kernels.
     int run_len = 0;
     size_t run_start = 0;
+    bool pending = false;
     for (int s = 0; s < rest; s++) {
       if (c->timing_stride > 0 && run_len == 0 && c->ev_used + 2 <= c->ev.size()) {
         run_start = c->ev_used;
@@ -1440,19 +1641,25 @@ int mcl_sweeps(void *ctx, int32_t nsweeps, int32_t measure_every) {
       } else if (run_len > 0) {
         run_len++;
       }
-      sf(grid, block, c->stream, *c, 0, (unsigned)s);
-      sf(grid, block, c->stream, *c, 1, (unsigned)s);
       const bool measure_now = measure_every > 0 && (s + 1) % measure_every == 0;
+      sf(grid, block, c->stream, *c, 0, (unsigned)s, false, pending ? 1 : 0);
+      pending = false;
+      sf(grid, block, c->stream, *c, 1, (unsigned)s, measure_now && c->fused, 0);
       if (run_len > 0 && (measure_now || run_len == c->timing_stride || s == rest - 1)) {
         MCL_HIP(hipEventRecord(c->ev[run_start + 1], c->stream));
         c->ev_runs.push_back({run_start, 2 * run_len});
         run_len = 0;
       }
       if (measure_now) {
-        const int st = run_measure(c, 1);
-        if (st) return st;
+        if (c->fused) {
+          pending = true;
+        } else {
+          const int st = run_measure(c, 1);
+          if (st) return st;
+        }
       }
     }
+    if (int st = flush_finalize(c, &pending)) return st;
     hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, c->stream, c->sweep_base, (unsigned)rest);
     c->sweeps_done += (unsigned long long)rest;
   }

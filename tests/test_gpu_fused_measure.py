@@ -1,0 +1,81 @@
+"""The measurement folded into the colour-1 sweep pass against the separate measurement kernels (needs an MI355X).
+
+After a measured sweep the moments hold one sample of the lattice that sweep left behind; mcl_measure then measures the
+same lattice with k_measure. The energies come from different FP32 site sums (colour-1 final energies in the sweep,
+colour-0 bond sums in k_measure), so they agree to FP32 rounding (1e-6 relative); S comes from the same directors
+summed in FP64 in a different order (1e-10). Cases cover the fully periodic path, the field term, a free x axis of odd
+length (the last column of colour 0 is counted from its -x side), the flip kernel and several lattices in one
+context.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def rand_dirs(n, seed):
+    rng = np.random.default_rng(seed)
+    v = rng.normal(size=(n, 3))
+    return (v / np.linalg.norm(v, axis=1, keepdims=True)).astype(np.float32).reshape(-1)
+
+
+def lattice(mcl, dims, periodic, model, seed, T=1.0):
+    nx, ny, nz = dims
+    return mcl.Lattice(nx, ny, nz, rand_dirs(nx * ny * nz, seed), periodic=periodic, model=model, T=T, seed=seed)
+
+
+CASES = {
+    "ll-periodic": [((16, 16, 16), (1, 1, 1), ("ll", {}))],
+    "ll-field": [((12, 10, 8), (1, 1, 1), ("ll", dict(elec_dir=(0.7071, 0.7071, 0.0), elecA=0.5)))],
+    "ghrl-free-odd-x": [((9, 8, 6), (0, 1, 0), ("ghrl", dict(ghrl=(0.7272726, -1.6363635, -1.0, -0.1818182, 0.0))))],
+    "ll-flip-free-z": [((10, 10, 10), (1, 1, 0), ("ll", dict(elec_dir=(0.0, 0.0, 1.0), elecA=0.3, elecP=0.2)))],
+    "ll-three-lattices": [((8, 8, 8), (1, 1, 1), ("ll", {})), ((16, 12, 10), (1, 1, 1), ("ll", {})),
+                          ((6, 14, 4), (1, 1, 1), ("ll", {}))],
+}
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_fused_measurement_equals_measure_kernel(mcl, name):
+    lats = [lattice(mcl, dims, per, mcl.make_model(pot, **kw), 100 + k)
+            for k, (dims, per, (pot, kw)) in enumerate(CASES[name])]
+    with mcl.Engine(lats) as eng:
+        eng.sweeps(3)
+        for rep in range(2):
+            eng.reset_moments()
+            eng.sweeps(1, 1)
+            for r in range(len(lats)):
+                m = eng.moments(r)
+                o = eng.measure(r)
+                assert m["n"] == 1
+                assert m["E"] == pytest.approx(o["E"], rel=1e-6, abs=1e-7), (name, r)
+                assert m["S"] == pytest.approx(o["S"], rel=1e-10, abs=1e-12), (name, r)
+                assert m["E2"] == pytest.approx(o["E"] ** 2, rel=2e-6)
+
+
+def run_bulk(mcl, shape=None, graph=0, calls=((40, 4), (7, 0), (24, 1))):
+    lats = [lattice(mcl, (12, 12, 12), (1, 1, 1), mcl.make_model("ll"), 200 + k, T=0.9 + 0.2 * k) for k in range(3)]
+    with mcl.Engine(lats) as eng:
+        if shape:
+            eng.set_launch(*shape)
+        if graph:
+            eng.set_graph(graph)
+        out = []
+        for n, every in calls:
+            eng.sweeps(n, every)
+            out.append(([eng.directors(r) for r in range(3)], [eng.moments(r) for r in range(3)]))
+        return out
+
+
+def test_fused_measurement_under_graphs_and_launch_shapes(mcl):
+    """Directors stay bit-identical across eager launches, graph replay and launch shapes; the moments, whose FP64
+    partial sums follow the sweep's workgroups, agree to 1e-12 (measurements finalized in the next sweep's extra
+    workgroups, in a trailing finalize, or inside a graph)."""
+    ref = run_bulk(mcl)
+    for kw in (dict(graph=8), dict(shape=(64, 0)), dict(shape=(1024, 1)), dict(shape=(128, 1), graph=12)):
+        got = run_bulk(mcl, **kw)
+        for (da, ma), (db, mb) in zip(ref, got):
+            assert all(np.array_equal(x, y) for x, y in zip(da, db)), kw
+            for a, b in zip(ma, mb):
+                assert a["n"] == b["n"]
+                for k in ("S", "S2", "E", "E2"):
+                    assert b[k] == pytest.approx(a[k], rel=1e-12, abs=1e-15), (kw, k)
