@@ -164,7 +164,9 @@ int mcl_philox(const uint32_t *ctr, const uint32_t *key, int32_t n, uint32_t *ou
 /* ---- timing (used by the benchmark) ---- */
 /* When stride > 0, mcl_sweeps records a HIP event pair around each run of consecutive sweep-kernel launches (at
  * most stride sweeps per run; a measurement ends a run), using events created up front for max_timed_sweeps
- * sweeps. mcl_kernel_timing returns the number of timed sweep-kernel launches and their summed duration
+ * sweeps. With graph replay on (mcl_set_graph), the pair goes around each replay instead, and the replay's duration
+ * counts for its 2 * len sweep-kernel launches (including the measurement work folded into them and the replay's
+ * final finalize). mcl_kernel_timing returns the number of timed sweep-kernel launches and their summed duration
  * (synchronizes). stride = 0 switches timing off. */
 int mcl_set_kernel_timing(void *ctx, int32_t stride, int32_t max_timed_sweeps);
 int mcl_kernel_timing(void *ctx, int64_t *launches, double *total_ms);

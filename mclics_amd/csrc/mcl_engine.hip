@@ -40,7 +40,7 @@ namespace {
 
 constexpr int kBlock = 256;          // threads per workgroup (4 wave64)
 constexpr int kWaves = kBlock / 64;
-constexpr int kFinBlock = 1024;  // finalize: one partial set per thread for up to 1024 workgroups of a lattice
+constexpr int kFinBlock = 256;  // threads of the finalize kernel
 constexpr int kFinWaves = kFinBlock / 64;
 constexpr int kMeasPerThread = 8;    // sites per thread in a measurement workgroup
 constexpr int kNQ = 24;              // FP64 quantities per measurement partial
@@ -390,7 +390,7 @@ constexpr int kNQS = 8;
 // Finalize of one lattice's compact partials [rg.x, rg.y) (column stride `stride`) by a whole workgroup of any
 // multiple of 64 threads: fixed-order sum, observables, and the sample added to the moments. The quantities of the
 // radial and pear observables are zero in this layout.
-__device__ void finalize_compact(int r, const double *__restrict__ part, int stride, int2 rg,
+__device__ __forceinline__ void finalize_compact(int r, const double *__restrict__ part, int stride, int2 rg,
                                  MomDev *__restrict__ mom, ObsDev *__restrict__ obs) {
   __shared__ double fred[16][kNQS];
   double v[kNQS];
@@ -476,6 +476,17 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8))) vo
   const int lane = threadIdx.x & 63;
   const int lblk = bd.y / ((FLIP ? 1 : 2) * (int)blockDim.x);  // workgroup index within the lattice
   unsigned int *const cnt_r = cnt + (size_t)r * kCntLattice;
+  // Sites per thread: without the flip trial a pair of positions (p, p + 64) of the colour block shares one
+  // Philox4x32-10 draw, 64 bits per site (16-bit angle, 16-bit axis choice, 32-bit acceptance uniform): each wave
+  // updates 128 consecutive positions, and each of its loads stays one contiguous 256-byte span. With the flip
+  // trial every site takes a draw of its own (a fourth 32-bit uniform). The counter (pair or position, sweep,
+  // tag + colour) depends on the position only, never on the launch shape (a workgroup spans a multiple of 128
+  // positions).
+  constexpr int SPT = FLIP ? 1 : 2;
+  const int tw = SPT == 2 ? bd.y + 128 * (int)(threadIdx.x >> 6) + lane : bd.y + (int)threadIdx.x;
+  const uint32_t ctr0 = SPT == 2 ? (uint32_t)(((bd.y >> 7) + (int)(threadIdx.x >> 6)) * 64 + lane) : (uint32_t)tw;
+  const mcl::Philox4 R = mcl::philox4x32_10(ctr0, (uint32_t)sweep, (uint32_t)(sweep >> 32),
+                                            kTagSweep + (uint32_t)colour, L.key0, L.key1);
   __shared__ float av_sh;
   float av;
   if (colour == 1) {
@@ -499,17 +510,6 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8))) vo
     if (threadIdx.x == 0) ctrl[r].av_store[sweep & 1] = av;
     cnt_r[((sweep + 1) % 3) * kCntSweep + threadIdx.x * kCntStride] = 0u;
   }
-  // Sites per thread: without the flip trial a pair of positions (p, p + 64) of the colour block shares one
-  // Philox4x32-10 draw, 64 bits per site (16-bit angle, 16-bit axis choice, 32-bit acceptance uniform): each wave
-  // updates 128 consecutive positions, and each of its loads stays one contiguous 256-byte span. With the flip
-  // trial every site takes a draw of its own (a fourth 32-bit uniform). The counter (pair or position, sweep,
-  // tag + colour) depends on the position only, never on the launch shape (a workgroup spans a multiple of 128
-  // positions).
-  constexpr int SPT = FLIP ? 1 : 2;
-  const int tw = SPT == 2 ? bd.y + 128 * (int)(threadIdx.x >> 6) + lane : bd.y + (int)threadIdx.x;
-  const uint32_t ctr0 = SPT == 2 ? (uint32_t)(((bd.y >> 7) + (int)(threadIdx.x >> 6)) * 64 + lane) : (uint32_t)tw;
-  const mcl::Philox4 R = mcl::philox4x32_10(ctr0, (uint32_t)sweep, (uint32_t)(sweep >> 32),
-                                            kTagSweep + (uint32_t)colour, L.key0, L.key1);
   unsigned nacc = 0;
   double qm[kNQS];
 #pragma unroll
@@ -526,10 +526,12 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8))) vo
   };
 #pragma unroll
   for (int e = 0; e < SPT; e++) {
-    const int t = tw + 64 * e;
     Site s;
     uint32_t self = 0;
-    if (t >= L.nc || !gather<MASK, SURF, FREE>(L, colour, t, X, Y, Z, PT, SX, SY, SZ, s, self)) continue;
+    {
+      const int t = tw + 64 * e;
+      if (t >= L.nc || !gather<MASK, SURF, FREE>(L, colour, t, X, Y, Z, PT, SX, SY, SZ, s, self)) continue;
+    }
     float u_ang, rot, u;
     if constexpr (SPT == 2) {
       const uint32_t w = R.v[2 * e];
@@ -788,20 +790,20 @@ __device__ __forceinline__ double eig_sym3_max(const double a[6], double v[3]) {
   rr = fmin(1.0, fmax(-1.0, rr));
   const double phi = acos(rr) / 3.0;
   lam = q + 2.0 * p * cos(phi);
-  const double r0[3] = {a00 - lam, a01, a02}, r1[3] = {a01, a11 - lam, a12}, r2[3] = {a02, a12, a22 - lam};
-  double c[3][3];
-  c[0][0] = r0[1] * r1[2] - r0[2] * r1[1]; c[0][1] = r0[2] * r1[0] - r0[0] * r1[2]; c[0][2] = r0[0] * r1[1] - r0[1] * r1[0];
-  c[1][0] = r0[1] * r2[2] - r0[2] * r2[1]; c[1][1] = r0[2] * r2[0] - r0[0] * r2[2]; c[1][2] = r0[0] * r2[1] - r0[1] * r2[0];
-  c[2][0] = r1[1] * r2[2] - r1[2] * r2[1]; c[2][1] = r1[2] * r2[0] - r1[0] * r2[2]; c[2][2] = r1[0] * r2[1] - r1[1] * r2[0];
-  int best = 0;
-  double bn = 0.0;
-  for (int x = 0; x < 3; x++) {
-    const double nn = c[x][0] * c[x][0] + c[x][1] * c[x][1] + c[x][2] * c[x][2];
-    if (nn > bn) { bn = nn; best = x; }
-  }
+  // cross products of the rows of A - lambda I (scalars, so nothing is indexed at run time)
+  const double r00 = a00 - lam, r11 = a11 - lam, r22 = a22 - lam;
+  const double c00 = a01 * a12 - a02 * r11, c01 = a02 * a01 - r00 * a12, c02 = r00 * r11 - a01 * a01;  // r0 x r1
+  const double c10 = a01 * r22 - a02 * a12, c11 = a02 * a02 - r00 * r22, c12 = r00 * a12 - a01 * a02;  // r0 x r2
+  const double c20 = r11 * r22 - a12 * a12, c21 = a12 * a02 - a01 * r22, c22 = a01 * a12 - r11 * a02;  // r1 x r2
+  const double n0 = c00 * c00 + c01 * c01 + c02 * c02;
+  const double n1 = c10 * c10 + c11 * c11 + c12 * c12;
+  const double n2 = c20 * c20 + c21 * c21 + c22 * c22;
+  double x = c00, y = c01, z = c02, bn = n0;
+  if (n1 > bn) { x = c10; y = c11; z = c12; bn = n1; }
+  if (n2 > bn) { x = c20; y = c21; z = c22; bn = n2; }
   if (bn > 0.0) {
     const double inv = 1.0 / sqrt(bn);
-    v[0] = c[best][0] * inv; v[1] = c[best][1] * inv; v[2] = c[best][2] * inv;
+    v[0] = x * inv; v[1] = y * inv; v[2] = z * inv;
   } else {
     v[0] = 0; v[1] = 0; v[2] = 1;
   }
@@ -1389,7 +1391,9 @@ int create_ctx(const mcl_lattice_desc *lat, int nlat, Ctx *c) {
   c->fused = !c->mask && !c->xq;
   for (int r = 0; r < nlat; r++)
     if (c->ld[r].nx < 2) c->fused = false;
-  if ((st = build_worklist(c, 256, 1))) return st;
+  // default launch shape (before any autotuning): 512 threads in dispatch order for a single lattice, 256 threads
+  // in XCD-contiguous tiles for a batch, the faster shapes measured for the 128^3 benchmark and the 512-lattice batch
+  if ((st = build_worklist(c, nlat == 1 ? 512 : 256, nlat == 1 ? 0 : 1))) return st;
   MCL_HIP(hipMemcpyAsync(c->mblk_d, mblk.data(), sizeof(int4) * mblk.size(), hipMemcpyHostToDevice, c->stream));
   std::vector<CtrlDev> ctrl(nlat);
   for (int r = 0; r < nlat; r++) {
@@ -1612,16 +1616,25 @@ int mcl_sweeps(void *ctx, int32_t nsweeps, int32_t measure_every) {
   const SweepFn sf = pick_sweep(*c);
   const dim3 grid = sweep_grid(c), block(c->bs);
   int done = 0;
-  if (c->graph_block > 0 && c->timing_stride == 0) {
+  if (c->graph_block > 0) {
     // whole graphs of len sweeps (a multiple of measure_every, so the measurements fall where the eager path
-    // puts them), then the rest eagerly
+    // puts them), then the rest eagerly; with kernel timing on, one event pair around each replay
     int len = c->graph_block;
     if (measure_every > 0) len = measure_every * std::max(1, (c->graph_block + measure_every / 2) / measure_every);
     if (nsweeps >= len) {
       hipGraphExec_t g;
       int st = get_graph(c, sf, len, measure_every, &g);
       if (st) return st;
-      for (; done + len <= nsweeps; done += len) MCL_HIP(hipGraphLaunch(g, c->stream));
+      for (; done + len <= nsweeps; done += len) {
+        const bool timed = c->timing_stride > 0 && c->ev_used + 2 <= c->ev.size();
+        if (timed) MCL_HIP(hipEventRecord(c->ev[c->ev_used], c->stream));
+        MCL_HIP(hipGraphLaunch(g, c->stream));
+        if (timed) {
+          MCL_HIP(hipEventRecord(c->ev[c->ev_used + 1], c->stream));
+          c->ev_runs.push_back({c->ev_used, 2 * len});
+          c->ev_used += 2;
+        }
+      }
       c->sweeps_done += (unsigned long long)done;
     }
   }
