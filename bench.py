@@ -49,13 +49,14 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def traffic_from_profiles(threads):
-    """HBM bytes per sweep-kernel launch at the given workgroup size, measured by rocprofv3 --pmc (FETCH_SIZE +
+def traffic_from_profiles(threads, xcd_order):
+    """HBM bytes per sweep-kernel launch at the given launch shape, measured by rocprofv3 --pmc (FETCH_SIZE +
     WRITE_SIZE, scaled by the copy kernel calibration), read from the committed summary if there is one."""
     path = os.path.join(ROOT, "profiles", "sweep_ll128_hbm_traffic.md")
     if not os.path.exists(path):
         return None
-    m = re.search(r"traffic_bytes_per_launch_%d:\s*([0-9.eE+]+)" % int(threads), open(path).read())
+    m = re.search(r"traffic_bytes_per_launch_%d_%d:\s*([0-9.eE+]+)" % (int(threads), int(xcd_order)),
+                  open(path).read())
     return float(m.group(1)) if m else None
 
 
@@ -321,7 +322,7 @@ def main():
                    "launch": tune, "graph_sweeps": a.graph,
                    "S_mean": round(S, 6), "E_mean": round(E, 6), "measurements": int(mom["n"])},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(tune["threads"]),
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(tune["threads"], tune["xcd_order"]),
                      "kernel": "k_sweep (one colour pass)", "bytes_per_launch": bytes_per_launch,
                      "avg_launch_us": round(avg_launch_s * 1e6, 3), "timed_launches": launches,
                      "copy_measured_gbs": None if copy_gbs is None else round(copy_gbs, 1)},
