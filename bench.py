@@ -23,6 +23,13 @@ Two further workloads print a line of the same form (the default run prints only
   --workload batch   512 Lebwohl-Lasher lattices in one context, sizes 16^3..64^3, bulk or homeotropic sphere
                      (W = 1.0), T uniform in [0.9, 1.3], per-lattice seeds, autotuned launch shape, MCT = 1000 and
                      MCS = 2000 with a measurement every 10 sweeps (graph replay is off while kernels are timed).
+  --workload refcases  the reference's own benchmark cases (paper/benchmark/run.sh:38-87): periodic bulk from a
+                     random IC (seed 1), LL (A = 1) at L = 32, 64, 128, 256 and T = 1.0, GHRL with the E7 constants
+                     at L = 64 and T = 0.1, run through the mclics program in thermalGPU mode with MCT = N, MCS = 1.
+                     As in the reference's make_table.py, the cost is the best wall time of the N-sweep runs minus
+                     the best of the 0-sweep runs (same setup, ic.csv and snapshot), so setup and output are left
+                     out; N is larger than the reference's so that the sweeps dominate on this GPU. One JSON line
+                     with a row per case, beside the reference's RTX 4070 figures from its results.csv.
 """
 import argparse
 import json
@@ -242,6 +249,62 @@ def cpu_baseline_batch(specs, sweeps=20):
                       "threads" % (sweeps, len(specs), tot_t, threads)}
 
 
+# (case, potential lines, L, T, N sweeps, reference RTX 4070 site-updates/ns from its results.csv)
+REF_CASES = (("ll32", ("potential ll", "A 1"), 32, 1.0, 1200000, 0.420),
+             ("ll64", ("potential ll", "A 1"), 64, 1.0, 300000, 0.962),
+             ("ll128", ("potential ll", "A 1"), 128, 1.0, 200000, 1.010),
+             ("ll256", ("potential ll", "A 1"), 256, 1.0, 40000, 1.154),
+             ("ghrl64", ("potential ghrl", "k11 1.48", "k22 1", "k33 2.28"), 64, 0.1, 200000, 1.043))
+
+
+def run_refcases(a, repeat=2):
+    """The reference's benchmark method on the mclics program (see the module docstring)."""
+    import shutil
+    import subprocess
+    import tempfile
+    exe = os.path.join(ROOT, "mclics_amd", "bin", "mclics")
+    if not os.path.exists(exe):
+        raise RuntimeError("mclics is not built (python -m mclics_amd.build)")
+    rows = []
+    work = tempfile.mkdtemp(prefix="mcl_refcases_")
+    try:
+        for name, pot, L, T, n, ref in REF_CASES:
+            best = {}
+            for rep in range(repeat):
+                for steps in (n, 0):
+                    d = os.path.join(work, "%s_%d" % (name, steps))
+                    os.makedirs(d, exist_ok=True)
+                    lines = ["Nx %d" % L, "Ny %d" % L, "Nz %d" % L, "geometry bulk", "xbound periodic",
+                             "ybound periodic", "zbound periodic"] + list(pot) + [
+                        "evol thermalGPU", "Ti %g" % T, "Tf %g" % T, "dT -0.1", "MCT %d" % steps, "MCS 1",
+                        "ic random", "seed 1"]
+                    with open(os.path.join(d, "input_parameters.txt"), "w") as f:
+                        f.write("\n".join(lines) + "\n")
+                    t0 = time.perf_counter()
+                    r = subprocess.run([exe, "input_parameters.txt"], cwd=d, stdout=subprocess.DEVNULL,
+                                       stderr=subprocess.PIPE, text=True)
+                    t = time.perf_counter() - t0
+                    if r.returncode != 0:
+                        raise RuntimeError("mclics failed on %s (%d sweeps): %s" % (name, steps, r.stderr[-500:]))
+                    best[steps] = min(best.get(steps, 1e30), t)
+                    shutil.rmtree(d)
+                    log("refcases: %s L=%d steps=%d %.2f s" % (name, L, steps, t))
+            cost = best[n] - best[0]
+            value = n * float(L) ** 3 / (cost * 1e9)
+            rows.append({"case": name, "L": L, "T": T, "steps": n, "t_steps_s": round(best[n], 3),
+                         "t_0_s": round(best[0], 3), "value": round(value, 3),
+                         "reference_rtx4070": ref, "ratio": round(value / ref, 1)})
+    finally:
+        shutil.rmtree(work, ignore_errors=True)
+    return {"metric": "site-updates/ns on the reference's benchmark cases (mclics thermalGPU, N-sweep minus "
+                      "0-sweep wall time)", "unit": "site-updates/ns", "n_gpus": 1, "higher_is_better": True,
+            "dtype": "f32", "data": "synthetic",
+            "config": {"workload": "paper/benchmark/run.sh cases: periodic bulk, random IC seed 1, LL L=32..256 "
+                                   "T=1.0, GHRL E7 L=64 T=0.1; best of %d" % repeat,
+                       "program_defaults": "graph replay of 50-sweep blocks, deterministic launch shape"},
+            "cases": rows}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -251,7 +314,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-bytes", type=int, default=256 << 20,
                     help="size of the STREAM-style copy that checks the HBM peak (0: skip)")
-    ap.add_argument("--workload", choices=("ll128", "slab", "batch"), default="ll128")
+    ap.add_argument("--workload", choices=("ll128", "slab", "batch", "refcases"), default="ll128")
     ap.add_argument("--graph", type=int, default=50,
                     help="sweeps per captured HIP graph in the 128^3 run (0: eager launches, events every "
                          "--timing-stride sweeps)")
@@ -265,7 +328,7 @@ def main():
     from mclics_amd import capi
     capi.load()
     if a.workload != "ll128":
-        out = run_slab(a) if a.workload == "slab" else run_batch_workload(a)
+        out = {"slab": run_slab, "batch": run_batch_workload, "refcases": run_refcases}[a.workload](a)
         print(json.dumps(out), flush=True)
         return
     nt = N ** 3

@@ -113,7 +113,10 @@ struct ObsDev {
 struct V3 {
   float x, y, z;
 };
-__device__ __forceinline__ float dot3(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+// The FP32 expressions of the trial are written with explicit fmaf wherever two products are summed, so that
+// every kernel instance rounds them the same way (the compiler's choice of which product to contract could
+// otherwise differ between instances, and trajectories must not depend on the instance).
+__device__ __forceinline__ float dot3(V3 a, V3 b) { return fmaf(a.z, b.z, fmaf(a.y, b.y, a.x * b.x)); }
 template <int AX>
 __device__ __forceinline__ float comp(V3 a) {
   return AX == 0 ? a.x : (AX == 1 ? a.y : a.z);
@@ -157,16 +160,17 @@ __device__ __forceinline__ float bond(const LatDev &L, V3 a, V3 b) {
     const float ai = SG * comp<AX>(a), aj = SG * comp<AX>(b);
     const float pij = 1.5f * nij * nij - 0.5f;
     // c = r.(n_j x n_i) (potential.cpp:50-52)
-    float cr = AX == 0 ? (a.z * b.y - a.y * b.z) : (AX == 1 ? (a.x * b.z - a.z * b.x) : (a.y * b.x - a.x * b.y));
+    float cr = AX == 0 ? fmaf(a.z, b.y, -(a.y * b.z))
+                       : (AX == 1 ? fmaf(a.x, b.z, -(a.z * b.x)) : fmaf(a.y, b.x, -(a.x * b.y)));
     cr *= SG;
-    const float E1 = 1.5f * (ai * ai + aj * aj) - 1.0f;
-    return E1 * (L.gr * pij + L.gl) + L.gm * (ai * aj * nij - (1.0f / 9.0f)) + L.gn * pij +
-           L.gs * (nij > 0.0f ? cr : -cr);
+    const float E1 = 1.5f * fmaf(aj, aj, ai * ai) - 1.0f;
+    const float e = fmaf(E1, fmaf(L.gr, pij, L.gl), L.gm * fmaf(ai * aj, nij, -(1.0f / 9.0f)));
+    return fmaf(L.gs, nij > 0.0f ? cr : -cr, fmaf(L.gn, pij, e));
   } else {
     // pear: -A b^2 - B1 b - (C/4) (1 + b)^2 r.(n_j - n_i)
     const float dr = SG * (comp<AX>(b) - comp<AX>(a));
     const float op = 1.0f + nij;
-    return -L.A * nij * nij - L.B1 * nij - (0.25f * L.C) * (op * op) * dr;
+    return fmaf(-(0.25f * L.C) * (op * op), dr, fmaf(-L.A * nij, nij, -(L.B1 * nij)));
   }
 }
 
@@ -211,13 +215,21 @@ struct Site {
 template <int POT>
 __device__ __forceinline__ float bond_sum(const LatDev &L, const Site &s, V3 n) {
   float e = 0.0f;
+  if constexpr (POT == MCL_POT_LL) {
+    // sum of nij^2 as a chain of fmaf (the same rounding whether or not the existence mask is a constant)
+#pragma unroll
+    for (int d = 0; d < 6; d++) {
+      const float q = dot3(n, s.nb[d]);
+      e = (s.ex & (1u << d)) ? fmaf(q, q, e) : e;
+    }
+    return e * (-1.5f * L.A);
+  }
   e += (s.ex & 1u) ? bond<POT, 0>(L, n, s.nb[0]) : 0.0f;
   e += (s.ex & 2u) ? bond<POT, 1>(L, n, s.nb[1]) : 0.0f;
   e += (s.ex & 4u) ? bond<POT, 2>(L, n, s.nb[2]) : 0.0f;
   e += (s.ex & 8u) ? bond<POT, 3>(L, n, s.nb[3]) : 0.0f;
   e += (s.ex & 16u) ? bond<POT, 4>(L, n, s.nb[4]) : 0.0f;
   e += (s.ex & 32u) ? bond<POT, 5>(L, n, s.nb[5]) : 0.0f;
-  if constexpr (POT == MCL_POT_LL) e *= -1.5f * L.A;
   return e;
 }
 
@@ -230,8 +242,8 @@ __device__ __forceinline__ float site_terms(const LatDev &L, int ptv, V3 sv, V3 
     if (ptv > 1) e += surface_energy(L, surf[L.surf0 + ptv - 2], n, sv);
   }
   if constexpr (FIELD) {
-    const float nE = elecE * (n.x * L.ex + n.y * L.ey + n.z * L.ez);
-    e += -L.elecP * nE - L.elecA * nE * nE;
+    const float nE = elecE * fmaf(n.z, L.ez, fmaf(n.y, L.ey, n.x * L.ex));
+    e += fmaf(-L.elecA * nE, nE, -(L.elecP * nE));
   }
   return e;
 }
@@ -320,6 +332,48 @@ __device__ __forceinline__ bool gather(const LatDev &L, int colour, int t, const
     if (s.ptv > 1) s.sv = V3{ldx(SX, self), ldx(SY, self), ldx(SZ, self)};
   }
   return true;
+}
+
+// gather<false, false, false> for a row-aligned lattice: every axis periodic, no mask, and nxh a multiple of 64
+// (nx a multiple of 128). The 64 positions [g, g + 64) of a lane group then lie in one row, so the row, j, k, the
+// parity of the row and the y / z wraps are wave-uniform (scalar registers): lane `lane` is position g + lane, and
+// only its x neighbours depend on the lane. Same sites, neighbours and order as gather().
+__device__ __forceinline__ void gather_row(const LatDev &L, int colour, uint32_t g, int lane,
+                                           const float *__restrict__ X, const float *__restrict__ Y,
+                                           const float *__restrict__ Z, Site &s, uint32_t &self) {
+  const uint32_t nxh = (uint32_t)L.nxh;
+  const uint32_t row = fdiv(g, L.mxh, L.sxh);
+  const uint32_t m0 = g - row * nxh;
+  const uint32_t k = fdiv(row, L.my, L.sy);
+  const uint32_t j = row - k * (uint32_t)L.ny;
+  const uint32_t par = (j + k + (uint32_t)colour) & 1u;
+  const uint32_t rowoff = L.off + row * nxh;
+  const uint32_t ob = rowoff + (uint32_t)((1 - colour) * L.nc);  // other colour, this row
+  const uint32_t m = m0 + (uint32_t)lane;
+  self = rowoff + (uint32_t)(colour * L.nc) + m;
+  s.i = (int)(2 * m + par);
+  s.j = (int)j;
+  s.k = (int)k;
+  s.ptv = 1;
+  s.ex = 63u;
+  s.sv = V3{0.0f, 0.0f, 0.0f};
+  s.n = V3{ldx(X, self), ldx(Y, self), ldx(Z, self)};
+  // parity 0 (i = 2m): +x neighbour at m, -x at m - 1; parity 1 (i = 2m + 1): +x at m + 1, -x at m
+  uint32_t xp, xm;
+  if (par) {
+    xp = ob + (m + 1 == nxh ? 0u : m + 1);
+    xm = ob + m;
+  } else {
+    xp = ob + m;
+    xm = ob + (m == 0 ? nxh - 1 : m - 1);
+  }
+  const uint32_t yp = (j + 1 == (uint32_t)L.ny ? ob - L.ywrap : ob + nxh) + m;
+  const uint32_t ym = (j == 0 ? ob + L.ywrap : ob - nxh) + m;
+  const uint32_t zp = (k + 1 == (uint32_t)L.nz ? ob - L.zwrap : ob + L.plane) + m;
+  const uint32_t zm = (k == 0 ? ob + L.zwrap : ob - L.plane) + m;
+  const uint32_t nidx[6] = {xp, xm, yp, ym, zp, zm};
+#pragma unroll
+  for (int d = 0; d < 6; d++) s.nb[d] = V3{ldx(X, nidx[d]), ldx(Y, nidx[d]), ldx(Z, nidx[d])};
 }
 
 // The site alone (no neighbours): decodes position t, loads its point type, director and anchoring vector.
@@ -438,7 +492,7 @@ __device__ __forceinline__ void finalize_compact(int r, const double *__restrict
   mom[r] = m;
 }
 
-template <int POT, bool MASK, bool SURF, bool FIELD, bool FLIP, bool FREE, bool MEAS>
+template <int POT, bool MASK, bool SURF, bool FIELD, bool FLIP, bool FREE, bool MEAS, bool ROW>
 // Launch shape (chosen by the autotuner): any multiple of 64 up to 1024 threads per workgroup; with xcd != 0
 // workgroup b runs logical tile (b & 7) * chunk + (b >> 3), so consecutive tiles share an XCD and its L2.
 // The sweep number is *sweep_base + k: graphs replay fixed arguments while the base advances on the device.
@@ -448,6 +502,8 @@ template <int POT, bool MASK, bool SURF, bool FIELD, bool FLIP, bool FREE, bool 
 // energy minus their one-site terms, plus the one-site terms of colour 0. Each colour-0 site is counted (Q tensor,
 // one-site terms, count) by the colour-1 site on its +x side, or, at the last column of a free x axis, by the one on
 // its -x side. The workgroup writes kNQS FP64 partials to spart[q * nblk + workgroup].
+// ROW: every lattice of the context is row-aligned (gather_row): site decoding and the y / z neighbour indices
+// are scalar work.
 __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_sweep(
     const LatDev *__restrict__ lats, CtrlDev *__restrict__ ctrl,
                                                 unsigned int *__restrict__ cnt, const int2 *__restrict__ blk,
@@ -461,6 +517,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8))) vo
                                                   const int2 *__restrict__ srange, MomDev *__restrict__ mom,
                                                   ObsDev *__restrict__ obs) {
   static_assert(!(MEAS && MASK), "the fused measurement needs unmasked lattices");
+  static_assert(!(ROW && (MASK || SURF || FREE)), "row-aligned gathering needs unmasked all-periodic lattices");
   const int b = blockIdx.x;
   const int Lb = xcd ? (b & 7) * chunk + (b >> 3) : b;
   if (Lb >= nblk) {
@@ -528,7 +585,13 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8))) vo
   for (int e = 0; e < SPT; e++) {
     Site s;
     uint32_t self = 0;
-    {
+    if constexpr (ROW) {
+      // first position of this lane group, wave-uniform (a wave covers 64 SPT consecutive positions)
+      const uint32_t g = (uint32_t)bd.y + (uint32_t)(64 * SPT * __builtin_amdgcn_readfirstlane(threadIdx.x >> 6)) +
+                         (uint32_t)(64 * e);
+      if (g >= (uint32_t)L.nc) continue;
+      gather_row(L, colour, g, lane, X, Y, Z, s, self);
+    } else {
       const int t = tw + 64 * e;
       if (t >= L.nc || !gather<MASK, SURF, FREE>(L, colour, t, X, Y, Z, PT, SX, SY, SZ, s, self)) continue;
     }
@@ -550,11 +613,11 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8))) vo
     const V3 n = s.n;
     V3 nn;
     if (rot < 0.333f) {
-      nn = V3{n.x, n.y * cs + n.z * sn, n.z * cs - n.y * sn};
+      nn = V3{n.x, fmaf(n.y, cs, n.z * sn), fmaf(n.z, cs, -(n.y * sn))};
     } else if (rot < 0.666f) {
-      nn = V3{n.x * cs - n.z * sn, n.y, n.z * cs + n.x * sn};
+      nn = V3{fmaf(n.x, cs, -(n.z * sn)), n.y, fmaf(n.z, cs, n.x * sn)};
     } else {
-      nn = V3{n.x * cs + n.y * sn, n.y * cs - n.x * sn, n.z};
+      nn = V3{fmaf(n.x, cs, n.y * sn), fmaf(n.y, cs, -(n.x * sn)), n.z};
     }
     const float inv = __builtin_amdgcn_rsqf(dot3(nn, nn));  // |nn| ~ 1: no denormal handling needed
     nn.x *= inv;
@@ -1029,6 +1092,7 @@ struct Ctx {
   int pot = 0;
   bool mask = false, surf = false, field = false, flip = false;
   bool free = false;  // some lattice has a free (non-periodic) axis
+  bool row = false;   // every lattice is row-aligned: all-periodic, unmasked, nx a multiple of 128 (gather_row)
   bool xq = false;  // radial or pear observables are measured
   std::vector<LatHost> lh;
   std::vector<LatDev> ld;
@@ -1069,19 +1133,31 @@ struct Ctx {
 using SweepFn = void (*)(dim3, dim3, hipStream_t, const Ctx &, int, unsigned, bool, int);
 using MeasFn = void (*)(dim3, dim3, hipStream_t, const Ctx &);
 
+template <int POT, bool MASK, bool SURF, bool FIELD, bool FLIP, bool FREE, bool MEAS, bool ROW>
+void launch_sweep_k(dim3 g, dim3 b, hipStream_t st, const Ctx &c, int colour, unsigned k, int fin) {
+  hipLaunchKernelGGL((k_sweep<POT, MASK, SURF, FIELD, FLIP, FREE, MEAS, ROW>), g, b, 0, st, c.lat_d, c.ctrl_d, c.cnt,
+                     c.blk_d, c.nblk, c.chunk, c.xcd, colour, c.sweep_base, k, c.X, c.Y, c.Z, c.PT, c.SX, c.SY, c.SZ,
+                     c.surf_d, c.spart, fin, c.nlat, c.sr_d, c.mom_d, c.obs_d);
+}
+
 template <int POT, bool MASK, bool SURF, bool FIELD, bool FLIP, bool FREE>
 void launch_sweep(dim3 g, dim3 b, hipStream_t st, const Ctx &c, int colour, unsigned k, bool meas, int fin) {
-  if constexpr (!MASK) {
-    if (meas) {
-      hipLaunchKernelGGL((k_sweep<POT, MASK, SURF, FIELD, FLIP, FREE, true>), g, b, 0, st, c.lat_d, c.ctrl_d, c.cnt,
-                         c.blk_d, c.nblk, c.chunk, c.xcd, colour, c.sweep_base, k, c.X, c.Y, c.Z, c.PT, c.SX, c.SY,
-                         c.SZ, c.surf_d, c.spart, fin, c.nlat, c.sr_d, c.mom_d, c.obs_d);
+  if constexpr (!MASK && !SURF && !FREE) {
+    if (c.row) {
+      if (meas)
+        launch_sweep_k<POT, MASK, SURF, FIELD, FLIP, FREE, true, true>(g, b, st, c, colour, k, fin);
+      else
+        launch_sweep_k<POT, MASK, SURF, FIELD, FLIP, FREE, false, true>(g, b, st, c, colour, k, fin);
       return;
     }
   }
-  hipLaunchKernelGGL((k_sweep<POT, MASK, SURF, FIELD, FLIP, FREE, false>), g, b, 0, st, c.lat_d, c.ctrl_d, c.cnt,
-                     c.blk_d, c.nblk, c.chunk, c.xcd, colour, c.sweep_base, k, c.X, c.Y, c.Z, c.PT, c.SX, c.SY, c.SZ,
-                     c.surf_d, c.spart, fin, c.nlat, c.sr_d, c.mom_d, c.obs_d);
+  if constexpr (!MASK) {
+    if (meas) {
+      launch_sweep_k<POT, MASK, SURF, FIELD, FLIP, FREE, true, false>(g, b, st, c, colour, k, fin);
+      return;
+    }
+  }
+  launch_sweep_k<POT, MASK, SURF, FIELD, FLIP, FREE, false, false>(g, b, st, c, colour, k, fin);
 }
 
 // Grid of a sweep launch: the work list, plus one workgroup per lattice for the deferred finalize of a measurement
@@ -1389,11 +1465,22 @@ int create_ctx(const mcl_lattice_desc *lat, int nlat, Ctx *c) {
   const unsigned long long zero_base = 0ull;
   MCL_HIP(hipMemcpyAsync(c->sweep_base, &zero_base, sizeof(zero_base), hipMemcpyHostToDevice, c->stream));
   c->fused = !c->mask && !c->xq;
-  for (int r = 0; r < nlat; r++)
+  c->row = !c->mask && !c->surf && !c->free;
+  for (int r = 0; r < nlat; r++) {
     if (c->ld[r].nx < 2) c->fused = false;
+    if (c->ld[r].nx % 128 != 0) c->row = false;
+  }
   // default launch shape (before any autotuning): 512 threads in dispatch order for a single lattice, 256 threads
-  // in XCD-contiguous tiles for a batch, the faster shapes measured for the 128^3 benchmark and the 512-lattice batch
-  if ((st = build_worklist(c, nlat == 1 ? 512 : 256, nlat == 1 ? 0 : 1))) return st;
+  // in XCD-contiguous tiles for a batch, the faster shapes measured for the 128^3 benchmark and the 512-lattice batch.
+  // A single lattice too small for 1024 workgroups of 512 threads takes smaller workgroups (down to 64 threads), so
+  // that its work spreads over more of the 256 CUs. Deterministic: the same lattice always gets the same shape.
+  int bs0 = 256;
+  if (nlat == 1) {
+    const long long span = c->flip ? 1 : 2;
+    bs0 = 512;
+    while (bs0 > 64 && (long long)c->ld[0].nc < 1024ll * span * bs0) bs0 >>= 1;
+  }
+  if ((st = build_worklist(c, bs0, nlat == 1 ? 0 : 1))) return st;
   MCL_HIP(hipMemcpyAsync(c->mblk_d, mblk.data(), sizeof(int4) * mblk.size(), hipMemcpyHostToDevice, c->stream));
   std::vector<CtrlDev> ctrl(nlat);
   for (int r = 0; r < nlat; r++) {

@@ -21,6 +21,15 @@ struct Philox4 {
 
 MCL_HD uint32_t mulhi32(uint32_t a, uint32_t b) { return (uint32_t)(((uint64_t)a * b) >> 32); }
 
+// a ^ b ^ c; on gfx950 one v_bitop3_b32 (truth table 0x96) instead of two v_xor_b32.
+MCL_HD uint32_t xor3(uint32_t a, uint32_t b, uint32_t c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
+#else
+  return a ^ b ^ c;
+#endif
+}
+
 // Ten rounds; round: (hi0, lo0) = M0 * c0, (hi1, lo1) = M1 * c2, c = (hi1 ^ c1 ^ k0, lo1, hi0 ^ c3 ^ k1, lo0);
 // the key advances by the Weyl constants between rounds. Each 32x32 -> 64-bit product is one multiply (on gfx950
 // a single v_mad_u64_u32 gives both halves, where a separate high and low multiply would cost two).
@@ -33,9 +42,9 @@ MCL_HD Philox4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
     const uint64_t p0 = (uint64_t)M0 * c0, p1 = (uint64_t)M1 * c2;
     const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
     const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
-    c0 = hi1 ^ c1 ^ k0;
+    c0 = xor3(hi1, c1, k0);
     c1 = lo1;
-    c2 = hi0 ^ c3 ^ k1;
+    c2 = xor3(hi0, c3, k1);
     c3 = lo0;
     k0 += W0;
     k1 += W1;

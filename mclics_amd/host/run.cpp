@@ -38,6 +38,13 @@ void check(int st, const char *what) {
   if (st != MCL_OK) gpu_fail(what);
 }
 
+// Environment switch: unset -> dflt, "0" -> false, anything else -> true.
+bool env_flag(const char *name, bool dflt) {
+  const char *v = getenv(name);
+  if (!v || !*v) return dflt;
+  return strcmp(v, "0") != 0;
+}
+
 struct Extras {
   bool pear, sph, cyl;
   void header(FILE *f) const {
@@ -100,6 +107,15 @@ GpuRun::GpuRun(Params &p, Lattice &L) : p_(p), L_(L) {
   d.radial = (iequals(p.radial, "sphere") || iequals(p.radial, "both") ? MCL_RADIAL_SPHERE : 0) |
              (iequals(p.radial, "cylinder") || iequals(p.radial, "both") ? MCL_RADIAL_CYLINDER : 0);
   check(mcl_create(&d, 1, &ctx_), "setup");
+  // Sweeps replay captured HIP graphs of 50-sweep blocks (bit-identical to launching each kernel; MCLICS_GRAPH=0
+  // launches eagerly). MCLICS_AUTOTUNE=1 times the library's launch shapes first and keeps the fastest: the
+  // directors stay bit-identical, but the FP64 sums of the measurements follow the workgroups, so their last bits
+  // can differ between runs that pick different shapes.
+  if (env_flag("MCLICS_GRAPH", true)) check(mcl_set_graph(ctx_, 50), "setup");
+  if (env_flag("MCLICS_AUTOTUNE", false)) {
+    mcl_tune_result r;
+    check(mcl_autotune(ctx_, 20, &r), "autotune");
+  }
 }
 
 GpuRun::~GpuRun() {
