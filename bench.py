@@ -86,6 +86,15 @@ def cpu_baseline(budget_s=15.0):
                       "threads" % (nsw, t, threads)}
 
 
+def pick_launch(eng, a, trial_sweeps):
+    """The autotuner's launch shape (the state is restored after its trials), or the one --launch fixes."""
+    if a.launch:
+        th, xo = (int(v) for v in a.launch.split(","))
+        eng.set_launch(th, xo)
+        return {"threads": th, "xcd_order": xo, "fixed": True}
+    return eng.autotune(trial_sweeps)
+
+
 def kernel_roofline(launches, kms, bytes_per_launch, copy_gbs):
     avg_launch_s = kms / launches / 1e3
     achieved = bytes_per_launch / avg_launch_s / 1e9
@@ -137,7 +146,7 @@ def run_slab(a):
     log("bench: slab %dx%dx%d GHRL, %d field values, %d sweeps" % (nx, ny, nz, len(fields), steps))
     rows = []
     with capi.Engine(lat) as eng:
-        tune = eng.autotune(10)
+        tune = pick_launch(eng, a, 10)
         eng.sweeps(a.warmup)
         eng.synchronize()
         eng.set_kernel_timing(a.timing_stride, (steps + a.timing_stride - 1) // a.timing_stride + len(fields) * 2)
@@ -197,7 +206,7 @@ def run_batch_workload(a):
     steps = mct + mcs
     log("bench: batch of %d lattices, %d valid sites, %d sweeps" % (len(lats), nvalid.sum(), steps))
     with capi.Engine(lats) as eng:
-        tune = eng.autotune(10)
+        tune = pick_launch(eng, a, 10)
         eng.sweeps(a.warmup)
         eng.synchronize()
         eng.set_kernel_timing(a.timing_stride, steps // a.timing_stride + 2 * (mcs // every) + 4)
@@ -315,6 +324,9 @@ def main():
     ap.add_argument("--copy-bytes", type=int, default=256 << 20,
                     help="size of the STREAM-style copy that checks the HBM peak (0: skip)")
     ap.add_argument("--workload", choices=("ll128", "slab", "batch", "refcases"), default="ll128")
+    ap.add_argument("--launch", default=None, metavar="THREADS,XCD_ORDER",
+                    help="fixed launch shape instead of the autotuner's choice (for profiling runs, whose tracer "
+                         "changes the timings the autotuner compares)")
     ap.add_argument("--graph", type=int, default=50,
                     help="sweeps per captured HIP graph in the 128^3 run (0: eager launches, events every "
                          "--timing-stride sweeps)")
@@ -337,7 +349,7 @@ def main():
                        ang_var=0.5, seed=1)
     log("bench: 128^3 LL periodic, T = %.2f, warmup %d sweeps" % (T_BENCH, a.warmup))
     with capi.Engine(lat) as eng:
-        tune = eng.autotune(20)  # launch shape from the library's autotuner (state is restored afterwards)
+        tune = pick_launch(eng, a, 20)
         log("bench: launch shape %s" % tune)
         if a.graph:
             eng.set_graph(a.graph)  # replay captured graphs; the events then bracket each replay
