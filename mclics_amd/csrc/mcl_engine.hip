@@ -409,11 +409,28 @@ __device__ __forceinline__ bool gather_self(const LatDev &L, int colour, int t, 
   return true;
 }
 
-// Sum of a double over the 64 lanes of a wave (every lane gets the total).
+// One DPP move of a double (both 32-bit halves) across the lanes of a row of 16.
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double x) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), CTRL, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+
+__device__ __forceinline__ double readlane_f64(double x, int l) {
+  return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(x), l),
+                          __builtin_amdgcn_readlane(__double2loint(x), l));
+}
+
+// Sum of a double over the 64 lanes of a wave, in a fixed order, without LDS: DPP butterflies inside each row of 16
+// lanes (quad_perm [1,0,3,2], quad_perm [2,3,0,1], row_ror 4, row_ror 8), then the four row sums read from lanes 0,
+// 16, 32 and 48. Every lane gets the total. All lanes of the wave must be active.
 __device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
+  v += dpp_f64<0xB1>(v);
+  v += dpp_f64<0x4E>(v);
+  v += dpp_f64<0x124>(v);
+  v += dpp_f64<0x128>(v);
+  return (readlane_f64(v, 0) + readlane_f64(v, 16)) + (readlane_f64(v, 32) + readlane_f64(v, 48));
 }
 
 // ang_var rule of evolve.cu:430-436. The reference compares the double quotient acc / nvalid with 0.5; for
@@ -521,12 +538,17 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8))) vo
   static_assert(!(MEAS && MASK), "the fused measurement needs unmasked lattices");
   static_assert(!(ROW && (MASK || SURF || FREE)), "row-aligned gathering needs unmasked all-periodic lattices");
   const int b = blockIdx.x;
-  const int Lb = xcd ? (b & 7) * chunk + (b >> 3) : b;
-  if (Lb >= nblk) {
-    // the workgroups past the work list finalize the measurement of the previous sweep (fin = 1), one per lattice
-    if (fin && Lb < nblk + nlat) finalize_compact(Lb - nblk, spart, nblk, srange[Lb - nblk], mom, obs);
+  // logical workgroups [0, nfin) finalize the measurement folded into the previous sweep (fin = 1), one per lattice:
+  // they come first in dispatch order, so their FP64 eigen-solve runs beside the sweep tiles instead of after them.
+  // nfin is the number of lattices when the context measures in the sweep, else 0.
+  const int nfin = nlat;
+  const int L0 = xcd ? (b & 7) * chunk + (b >> 3) : b;
+  if (L0 < nfin) {
+    if (fin) finalize_compact(L0, spart, nblk, srange[L0], mom, obs);
     return;
   }
+  const int Lb = L0 - nfin;
+  if (Lb >= nblk) return;
   const int2 bd = blk[Lb];
   const int r = bd.x;
   const LatDev &L = lats[r];
@@ -546,22 +568,18 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8))) vo
   const uint32_t ctr0 = SPT == 2 ? (uint32_t)(((bd.y >> 7) + (int)(threadIdx.x >> 6)) * 64 + lane) : (uint32_t)tw;
   const mcl::Philox4 R = mcl::philox4x32_10(ctr0, (uint32_t)sweep, (uint32_t)(sweep >> 32),
                                             kTagSweep + (uint32_t)colour, L.key0, L.key1);
-  __shared__ float av_sh;
   float av;
   if (colour == 1) {
     av = ctrl[r].av_store[sweep & 1];
   } else if (sweep == ctrl[r].av_set_sweep) {
     av = ctrl[r].av_set;
   } else {
-    // accepted rotations of the previous sweep: wave 0 sums its 64 counters and shares the ang_var rule's result
-    if (threadIdx.x < 64) {
-      unsigned a = cnt_r[((sweep - 1) % 3) * kCntSweep + lane * kCntStride];
+    // accepted rotations of the previous sweep: every wave sums the 64 counters itself and applies the ang_var
+    // rule, so no wave waits at a barrier before issuing its loads (the counter load overlaps the gather)
+    unsigned a = cnt_r[((sweep - 1) % 3) * kCntSweep + lane * kCntStride];
 #pragma unroll
-      for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o, 64);
-      if (threadIdx.x == 0) av_sh = ang_var_rule(ctrl[r].av_store[(sweep - 1) & 1], a, L.nvalid);
-    }
-    __syncthreads();
-    av = av_sh;
+    for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o, 64);
+    av = ang_var_rule(ctrl[r].av_store[(sweep - 1) & 1], (unsigned)__builtin_amdgcn_readfirstlane(a), L.nvalid);
   }
   if (colour == 0 && lblk == 0 && threadIdx.x < kCnt) {
     // the lattice's first workgroup publishes ang_var for colour 1 and the next sweep, and clears the counters
@@ -1139,7 +1157,7 @@ template <int POT, bool MASK, bool SURF, bool FIELD, bool FLIP, bool FREE, bool 
 void launch_sweep_k(dim3 g, dim3 b, hipStream_t st, const Ctx &c, int colour, unsigned k, int fin) {
   hipLaunchKernelGGL((k_sweep<POT, MASK, SURF, FIELD, FLIP, FREE, MEAS, ROW>), g, b, 0, st, c.lat_d, c.ctrl_d, c.cnt,
                      c.blk_d, c.nblk, c.chunk, c.xcd, colour, c.sweep_base, k, c.X, c.Y, c.Z, c.PT, c.SX, c.SY, c.SZ,
-                     c.surf_d, c.spart, fin, c.nlat, c.sr_d, c.mom_d, c.obs_d);
+                     c.surf_d, c.spart, fin, c.fused ? c.nlat : 0, c.sr_d, c.mom_d, c.obs_d);
 }
 
 template <int POT, bool MASK, bool SURF, bool FIELD, bool FLIP, bool FREE>
