@@ -44,7 +44,7 @@ def build_host():
     tmp = BIN + ".tmp"
     _run(["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"), "-I",
           os.path.join(HERE, "csrc"), "-o", tmp] + srcs +
-         ["-L", os.path.dirname(LIB), "-lmclgpu", "-Wl,-rpath,$ORIGIN/../lib"])
+         ["-L", os.path.dirname(LIB), "-lmclgpu", "-Wl,-rpath,$ORIGIN/../lib", "-pthread"])
     os.replace(tmp, BIN)
 
 

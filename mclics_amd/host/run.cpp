@@ -4,13 +4,17 @@
 // measurements (FP64 on the device) and the local order parameter of the snapshots run in libmclgpu.
 #include "run.h"
 
+#include "csvfmt.h"
+
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 #include <strings.h>
 
+#include <algorithm>
 #include <chrono>
+#include <thread>
 #include <string>
 #include <vector>
 
@@ -132,7 +136,9 @@ void GpuRun::sweeps(int n, int measure_every) {
 }
 
 // Snapshot and ic.csv format (io.cpp:56-89): header x,y,z,nx,ny,nz,S,pt, x fastest, %g, S = local order
-// parameter, NaN on empty sites.
+// parameter, NaN on empty sites. The local order parameter comes from the GPU (mcl_local_order); the text is
+// formatted by up to 16 threads, each on its own chunk of lines, and the chunks are written in order, so the file is
+// byte-identical to printing the lines one by one.
 int GpuRun::print_n(const char *fname) {
   const size_t Nt = L_.pt.size();
   std::vector<float> S(Nt);
@@ -143,15 +149,45 @@ int GpuRun::print_n(const char *fname) {
     perror(fname);
     return 1;
   }
-  std::vector<char> buf(1 << 20);
-  setvbuf(out, buf.data(), _IOFBF, buf.size());
-  fprintf(out, "x,y,z,nx,ny,nz,S,pt\n");
-  size_t s = 0;
-  for (int k = 0; k < L_.Nz; k++)
-    for (int j = 0; j < L_.Ny; j++)
-      for (int i = 0; i < L_.Nx; i++, s++)
-        fprintf(out, "%d,%d,%d,%g,%g,%g,%g,%d\n", i, j, k, L_.ni[3 * s], L_.ni[3 * s + 1], L_.ni[3 * s + 2],
-                L_.pt[s] ? S[s] : NAN, L_.pt[s]);
+  fputs("x,y,z,nx,ny,nz,S,pt\n", out);
+  const size_t kChunk = 65536;  // lines per chunk
+  const int nthreads = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  std::vector<std::vector<char>> bufs(nthreads, std::vector<char>(kChunk * mclcsv::kMaxLine));
+  std::vector<size_t> used(nthreads);
+  const int Nx = L_.Nx, Ny = L_.Ny;
+  auto format = [&](int t, size_t s0, size_t s1) {
+    char *p = bufs[t].data();
+    int i = (int)(s0 % Nx), j = (int)((s0 / Nx) % Ny), k = (int)(s0 / ((size_t)Nx * Ny));
+    for (size_t s = s0; s < s1; s++) {
+      const int pt = L_.pt[s];
+      p = mclcsv::put_line(p, i, j, k, L_.ni[3 * s], L_.ni[3 * s + 1], L_.ni[3 * s + 2], pt ? S[s] : NAN, pt);
+      if (++i == Nx) {
+        i = 0;
+        if (++j == Ny) {
+          j = 0;
+          k++;
+        }
+      }
+    }
+    used[t] = (size_t)(p - bufs[t].data());
+  };
+  for (size_t b0 = 0; b0 < Nt; b0 += kChunk * nthreads) {
+    std::vector<std::thread> pool;
+    int n = 0;
+    for (; n < nthreads && b0 + n * kChunk < Nt; n++) {
+      const size_t s0 = b0 + n * kChunk, s1 = std::min(Nt, s0 + kChunk);
+      if (n == 0) continue;  // chunk 0 is formatted by this thread below
+      pool.emplace_back(format, n, s0, s1);
+    }
+    format(0, b0, std::min(Nt, b0 + kChunk));
+    for (auto &th : pool) th.join();
+    for (int t = 0; t < n; t++)
+      if (fwrite(bufs[t].data(), 1, used[t], out) != used[t]) {
+        perror(fname);
+        fclose(out);
+        return 1;
+      }
+  }
   printf("  Snapshot -> %s\n", fname);
   fflush(stdout);
   fclose(out);
