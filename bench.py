@@ -86,13 +86,14 @@ def cpu_baseline(budget_s=15.0):
                       "threads" % (nsw, t, threads)}
 
 
-def pick_launch(eng, a, trial_sweeps):
-    """The autotuner's launch shape (the state is restored after its trials), or the one --launch fixes."""
+def pick_launch(eng, a, trial_sweeps, measure_every):
+    """The autotuner's launch shape for trials measured at the workload's cadence (the state is restored after the
+    trials), or the one --launch fixes."""
     if a.launch:
         th, xo = (int(v) for v in a.launch.split(","))
         eng.set_launch(th, xo)
         return {"threads": th, "xcd_order": xo, "fixed": True}
-    return eng.autotune(trial_sweeps)
+    return eng.autotune(trial_sweeps, measure_every)
 
 
 def kernel_roofline(launches, kms, bytes_per_launch, copy_gbs):
@@ -146,7 +147,7 @@ def run_slab(a):
     log("bench: slab %dx%dx%d GHRL, %d field values, %d sweeps" % (nx, ny, nz, len(fields), steps))
     rows = []
     with capi.Engine(lat) as eng:
-        tune = pick_launch(eng, a, 10)
+        tune = pick_launch(eng, a, 12, 3)  # the electric mode measures one sweep in three (MCS of MCT + MCS)
         eng.sweeps(a.warmup)
         eng.synchronize()
         eng.set_kernel_timing(a.timing_stride, (steps + a.timing_stride - 1) // a.timing_stride + len(fields) * 2)
@@ -206,7 +207,7 @@ def run_batch_workload(a):
     steps = mct + mcs
     log("bench: batch of %d lattices, %d valid sites, %d sweeps" % (len(lats), nvalid.sum(), steps))
     with capi.Engine(lats) as eng:
-        tune = pick_launch(eng, a, 10)
+        tune = pick_launch(eng, a, 10, MEASURE_EVERY)
         eng.sweeps(a.warmup)
         eng.synchronize()
         eng.set_kernel_timing(a.timing_stride, steps // a.timing_stride + 2 * (mcs // every) + 4)
@@ -349,7 +350,7 @@ def main():
                        ang_var=0.5, seed=1)
     log("bench: 128^3 LL periodic, T = %.2f, warmup %d sweeps" % (T_BENCH, a.warmup))
     with capi.Engine(lat) as eng:
-        tune = pick_launch(eng, a, 20)
+        tune = pick_launch(eng, a, 20, MEASURE_EVERY)
         log("bench: launch shape %s" % tune)
         if a.graph:
             eng.set_graph(a.graph)  # replay captured graphs; the events then bracket each replay

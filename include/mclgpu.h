@@ -191,6 +191,10 @@ typedef struct mcl_tune_result {
  * number) is saved before and restored after the trials. Results are cached per shape signature in the process
  * and, when the environment variable MCL_TUNE_CACHE names a file, in that file. */
 int mcl_autotune(void *ctx, int32_t trial_sweeps, mcl_tune_result *out);
+/* As mcl_autotune, with the trials measured every measure_every sweeps (0: never), the cadence of the run that
+ * follows: the measuring passes prefer other shapes than the plain ones (they hold more registers). The moments are
+ * saved and restored as well; results are cached per shape signature and cadence. */
+int mcl_autotune_ex(void *ctx, int32_t trial_sweeps, int32_t measure_every, mcl_tune_result *out);
 /* Sets the launch shape by hand (threads a multiple of 64, at most 1024). */
 int mcl_set_launch(void *ctx, int32_t threads, int32_t xcd_order);
 /* block_sweeps > 0: mcl_sweeps replays captured HIP graphs of about block_sweeps sweeps (a multiple of

@@ -108,7 +108,8 @@ def run_batch(specs, mct, mcs, every=1, autotune=True, graph=32, log=None):
     info = {}
     with capi.Engine(lats) as eng:
         if autotune:
-            info["tune"] = eng.autotune(10)
+            # trials at the run's cadence: mct plain sweeps, then one measurement every `every` sweeps
+            info["tune"] = eng.autotune(10, max(1, int(round(every * (mct + mcs) / max(1, mcs)))) if mcs else 0)
         if graph:
             eng.set_graph(graph)
         eng.synchronize()

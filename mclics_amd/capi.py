@@ -96,6 +96,7 @@ ABI = {
                                          ctypes.POINTER(ctypes.c_double)]),
     "mcl_get_stream": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)]),
     "mcl_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(TuneResult)]),
+    "mcl_autotune_ex": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(TuneResult)]),
     "mcl_set_launch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]),
     "mcl_set_graph": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "mcl_mlp_create": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_uint64,
@@ -303,9 +304,10 @@ class Engine:
                                           len(s), _fptr(ec), _fptr(et)))
         return ec, et
 
-    def autotune(self, trial_sweeps=20):
+    def autotune(self, trial_sweeps=20, measure_every=0):
+        """mcl_autotune_ex: the fastest launch shape for trial sweeps measured every measure_every sweeps."""
         r = TuneResult()
-        _check(self.lib.mcl_autotune(self.h, int(trial_sweeps), ctypes.byref(r)))
+        _check(self.lib.mcl_autotune_ex(self.h, int(trial_sweeps), int(measure_every), ctypes.byref(r)))
         return {"threads": r.threads, "xcd_order": r.xcd_order, "us_per_sweep": r.us_per_sweep,
                 "from_cache": bool(r.from_cache)}
 

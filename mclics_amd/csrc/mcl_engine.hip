@@ -211,6 +211,7 @@ struct Site {
   unsigned ex;  // bit d: neighbour d exists and is not empty
   int ptv;
   int i, j, k;
+  uint32_t qx[2];  // NBX: pool indices of the +x and -x neighbours
 };
 
 // Bond part of the site energy: the bonds to the existing neighbours (LL scaled by -1.5 A).
@@ -263,7 +264,9 @@ __device__ __forceinline__ float site_energy(const LatDev &L, const Site &s, V3 
 // Returns false for padding positions (i >= nx) and empty sites.
 // FREE = false: every axis of every lattice of the context is periodic (and MASK is false), so every neighbour
 // exists and the wrap is the only boundary case.
-template <bool MASK, bool SURF, bool FREE = true>
+// NBX (measuring kernels of masked lattices): also keeps the pool indices of the two x neighbours, whose one-site
+// terms the measurement adds for the colour-0 sites a colour-1 site counts.
+template <bool MASK, bool SURF, bool FREE = true, bool NBX = false>
 __device__ __forceinline__ bool gather(const LatDev &L, int colour, int t, const float *__restrict__ X,
                                        const float *__restrict__ Y, const float *__restrict__ Z,
                                        const uint8_t *__restrict__ PT, const float *__restrict__ SX,
@@ -326,6 +329,9 @@ __device__ __forceinline__ bool gather(const LatDev &L, int colour, int t, const
     s.nb[d] = V3{ldx(X, q), ldx(Y, q), ldx(Z, q)};
     if constexpr (MASK) {
       if (ldx(PT, q) == 0) ex &= ~(1u << d);
+      if constexpr (NBX) {
+        if (d < 2) s.qx[d] = q;
+      }
     }
   }
   s.ex = ex;
@@ -523,7 +529,7 @@ template <int POT, bool MASK, bool SURF, bool FIELD, bool FLIP, bool FREE, bool 
 // its -x side. The workgroup writes kNQS FP64 partials to spart[q * nblk + workgroup].
 // ROW: every lattice of the context is row-aligned (gather_row): site decoding and the y / z neighbour indices
 // are scalar work.
-__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_sweep(
+__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(MEAS && MASK ? 6 : 8, 8))) void k_sweep(
     const LatDev *__restrict__ lats, CtrlDev *__restrict__ ctrl,
                                                 unsigned int *__restrict__ cnt, const int2 *__restrict__ blk,
                                                 int nblk, int chunk, int xcd, int colour,
@@ -535,7 +541,6 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8))) vo
                                                   double *__restrict__ spart, int fin, int nlat,
                                                   const int2 *__restrict__ srange, MomDev *__restrict__ mom,
                                                   ObsDev *__restrict__ obs) {
-  static_assert(!(MEAS && MASK), "the fused measurement needs unmasked lattices");
   static_assert(!(ROW && (MASK || SURF || FREE)), "row-aligned gathering needs unmasked all-periodic lattices");
   const int b = blockIdx.x;
   // logical workgroups [0, nfin) finalize the measurement folded into the previous sweep (fin = 1), one per lattice:
@@ -613,7 +618,8 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8))) vo
       gather_row(L, colour, g, lane, X, Y, Z, s, self);
     } else {
       const int t = tw + 64 * e;
-      if (t >= L.nc || !gather<MASK, SURF, FREE>(L, colour, t, X, Y, Z, PT, SX, SY, SZ, s, self)) continue;
+      if (t >= L.nc || !gather<MASK, SURF, FREE, MEAS && MASK>(L, colour, t, X, Y, Z, PT, SX, SY, SZ, s, self))
+        continue;
     }
     float u_ang, rot, u;
     if constexpr (SPT == 2) {
@@ -668,19 +674,38 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8))) vo
       stx(Z, self, nc.z);
     }
     if constexpr (MEAS) {
+      // one-site terms (anchoring on surface sites, field) of a site of point type pt and anchoring vector sv
+      auto terms = [&](int pt, V3 sv, V3 v) -> double {
+        if constexpr (SURF || FIELD) return (double)site_terms<SURF, FIELD>(L, pt, sv, v, surf, elecE);
+        return 0.0;
+      };
+      // the same for the x neighbour d of a masked lattice, whose point type and anchoring vector are loaded here
+      auto nb_terms = [&](int d) -> double {
+        if constexpr (MASK) {
+          const uint32_t q = s.qx[d];
+          const int pt = ldx(PT, q);
+          V3 sv{0.0f, 0.0f, 0.0f};
+          if constexpr (SURF) {
+            if (pt > 1) sv = V3{ldx(SX, q), ldx(SY, q), ldx(SZ, q)};
+          }
+          return terms(pt, sv, s.nb[d]);
+        }
+        return terms(1, V3{0.0f, 0.0f, 0.0f}, s.nb[d]);
+      };
       double e2 = 2.0 * (double)E_fin;
       add_q(nc);
-      if constexpr (FIELD) e2 -= (double)site_terms<false, true>(L, 1, V3{0.0f, 0.0f, 0.0f}, nc, surf, elecE);
+      e2 -= terms(s.ptv, s.sv, nc);
       // the colour-0 sites this site counts: its -x neighbour, and its +x neighbour at the end of a free x axis
+      // (masked lattices take this path only when none of their sites is empty, so every colour-0 site has them)
       const bool own_m = FREE ? (s.ex & 2u) != 0 : true;
       const bool own_p = FREE && !(L.flags & F_PX) && s.i + 1 == L.nx - 1;
       if (own_m) {
         add_q(s.nb[1]);
-        if constexpr (FIELD) e2 += (double)site_terms<false, true>(L, 1, V3{0.0f, 0.0f, 0.0f}, s.nb[1], surf, elecE);
+        e2 += nb_terms(1);
       }
       if (own_p) {
         add_q(s.nb[0]);
-        if constexpr (FIELD) e2 += (double)site_terms<false, true>(L, 1, V3{0.0f, 0.0f, 0.0f}, s.nb[0], surf, elecE);
+        e2 += nb_terms(0);
       }
       qm[0] += e2;
     }
@@ -1171,11 +1196,9 @@ void launch_sweep(dim3 g, dim3 b, hipStream_t st, const Ctx &c, int colour, unsi
       return;
     }
   }
-  if constexpr (!MASK) {
-    if (meas) {
-      launch_sweep_k<POT, MASK, SURF, FIELD, FLIP, FREE, true, false>(g, b, st, c, colour, k, fin);
-      return;
-    }
+  if (meas) {
+    launch_sweep_k<POT, MASK, SURF, FIELD, FLIP, FREE, true, false>(g, b, st, c, colour, k, fin);
+    return;
   }
   launch_sweep_k<POT, MASK, SURF, FIELD, FLIP, FREE, false, false>(g, b, st, c, colour, k, fin);
 }
@@ -1484,10 +1507,13 @@ int create_ctx(const mcl_lattice_desc *lat, int nlat, Ctx *c) {
   MCL_HIP(hipMemcpyAsync(c->lat_d, c->ld.data(), sizeof(LatDev) * nlat, hipMemcpyHostToDevice, c->stream));
   const unsigned long long zero_base = 0ull;
   MCL_HIP(hipMemcpyAsync(c->sweep_base, &zero_base, sizeof(zero_base), hipMemcpyHostToDevice, c->stream));
-  c->fused = !c->mask && !c->xq;
+  // the measurement is folded into the sweep unless radial or pear observables are measured, some lattice is a
+  // single column in x, or some lattice has empty sites (each colour-0 site is counted by an x neighbour)
+  c->fused = !c->xq;
   c->row = !c->mask && !c->surf && !c->free;
   for (int r = 0; r < nlat; r++) {
     if (c->ld[r].nx < 2) c->fused = false;
+    if ((long long)c->ld[r].nvalid != (long long)c->ld[r].nx * c->ld[r].ny * c->ld[r].nz) c->fused = false;
     if (c->ld[r].nx % 128 != 0) c->row = false;
   }
   // default launch shape (before any autotuning): 512 threads in dispatch order for a single lattice, 256 threads
@@ -2192,10 +2218,15 @@ extern "C" {
 // own lattices and keeps the fastest. The lattices, control blocks, counters and sweep count are saved before the
 // trials and restored after them, so tuning leaves the simulation state untouched.
 int mcl_autotune(void *ctx, int32_t trial_sweeps, mcl_tune_result *out) {
+  return mcl_autotune_ex(ctx, trial_sweeps, 0, out);
+}
+
+int mcl_autotune_ex(void *ctx, int32_t trial_sweeps, int32_t measure_every, mcl_tune_result *out) {
   Ctx *c = as_ctx(ctx);
   if (!c || !out) return fail(MCL_ERR_ARG, "null argument");
   if (trial_sweeps < 1) return fail(MCL_ERR_ARG, "trial_sweeps must be >= 1");
-  const std::string key = tune_key(c);
+  if (measure_every < 0) return fail(MCL_ERR_ARG, "measure_every must be >= 0");
+  const std::string key = tune_key(c) + ":every" + std::to_string(measure_every);
   mcl_tune_result best{};
   if (tune_lookup(key, &best)) {
     best.from_cache = 1;
@@ -2209,6 +2240,8 @@ int mcl_autotune(void *ctx, int32_t trial_sweeps, mcl_tune_result *out) {
   float *bx = nullptr, *by = nullptr, *bz = nullptr;
   CtrlDev *bctrl = nullptr;
   unsigned *bcnt = nullptr;
+  MomDev *bmom = nullptr;
+  ObsDev *bobs = nullptr;
   int st;
   const size_t pool = (size_t)c->pool, ncnt = (size_t)c->nlat * kCntLattice;
   auto release = [&]() {
@@ -2217,9 +2250,12 @@ int mcl_autotune(void *ctx, int32_t trial_sweeps, mcl_tune_result *out) {
     (void)hipFree(bz);
     (void)hipFree(bctrl);
     (void)hipFree(bcnt);
+    (void)hipFree(bmom);
+    (void)hipFree(bobs);
   };
   if ((st = dev_alloc(&bx, pool)) || (st = dev_alloc(&by, pool)) || (st = dev_alloc(&bz, pool)) ||
-      (st = dev_alloc(&bctrl, c->nlat)) || (st = dev_alloc(&bcnt, ncnt))) {
+      (st = dev_alloc(&bctrl, c->nlat)) || (st = dev_alloc(&bcnt, ncnt)) || (st = dev_alloc(&bmom, c->nlat)) ||
+      (st = dev_alloc(&bobs, c->nlat))) {
     release();
     return st;
   }
@@ -2229,6 +2265,8 @@ int mcl_autotune(void *ctx, int32_t trial_sweeps, mcl_tune_result *out) {
   if (e == hipSuccess)
     e = hipMemcpyAsync(bctrl, c->ctrl_d, sizeof(CtrlDev) * c->nlat, hipMemcpyDeviceToDevice, c->stream);
   if (e == hipSuccess) e = hipMemcpyAsync(bcnt, c->cnt, sizeof(unsigned) * ncnt, hipMemcpyDeviceToDevice, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(bmom, c->mom_d, sizeof(MomDev) * c->nlat, hipMemcpyDeviceToDevice, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(bobs, c->obs_d, sizeof(ObsDev) * c->nlat, hipMemcpyDeviceToDevice, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   if (e != hipSuccess) {
     release();
@@ -2248,7 +2286,7 @@ int mcl_autotune(void *ctx, int32_t trial_sweeps, mcl_tune_result *out) {
       if ((st = build_worklist(c, th, xo))) break;
       if ((st = mcl_sweeps(c, 2, 0))) break;  // warm-up
       if ((e = hipEventRecord(a, c->stream)) != hipSuccess) break;
-      if ((st = mcl_sweeps(c, trial_sweeps, 0))) break;
+      if ((st = mcl_sweeps(c, trial_sweeps, measure_every))) break;
       if ((e = hipEventRecord(b, c->stream)) != hipSuccess) break;
       if ((e = hipEventSynchronize(b)) != hipSuccess) break;
       float ms = 0.0f;
@@ -2271,6 +2309,8 @@ int mcl_autotune(void *ctx, int32_t trial_sweeps, mcl_tune_result *out) {
   if (r == hipSuccess)
     r = hipMemcpyAsync(c->ctrl_d, bctrl, sizeof(CtrlDev) * c->nlat, hipMemcpyDeviceToDevice, c->stream);
   if (r == hipSuccess) r = hipMemcpyAsync(c->cnt, bcnt, sizeof(unsigned) * ncnt, hipMemcpyDeviceToDevice, c->stream);
+  if (r == hipSuccess) r = hipMemcpyAsync(c->mom_d, bmom, sizeof(MomDev) * c->nlat, hipMemcpyDeviceToDevice, c->stream);
+  if (r == hipSuccess) r = hipMemcpyAsync(c->obs_d, bobs, sizeof(ObsDev) * c->nlat, hipMemcpyDeviceToDevice, c->stream);
   if (r == hipSuccess) r = hipMemcpyAsync(c->sweep_base, &base, sizeof(base), hipMemcpyHostToDevice, c->stream);
   if (r == hipSuccess) r = hipStreamSynchronize(c->stream);
   release();

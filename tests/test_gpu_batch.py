@@ -86,6 +86,14 @@ def test_autotune_keeps_the_state(mcl):
         eng.sweeps(20, 4)
         twin.sweeps(20, 4)
         assert same(state(eng, len(SPECS)), state(twin, len(SPECS)))
+        # trials measured every 2 sweeps (mcl_autotune_ex): the moments they add are restored as well
+        before = state(eng, len(SPECS))
+        res = eng.autotune(6, 2)
+        assert res["threads"] in (64, 128, 256, 512, 1024) and not res["from_cache"]
+        assert same(before, state(eng, len(SPECS)))
+        eng.sweeps(9, 3)
+        twin.sweeps(9, 3)
+        assert same(state(eng, len(SPECS)), state(twin, len(SPECS)))
 
 
 def test_small_batch_physics(mcl):

@@ -79,3 +79,41 @@ def test_fused_measurement_under_graphs_and_launch_shapes(mcl):
                 assert a["n"] == b["n"]
                 for k in ("S", "S2", "E", "E2"):
                     assert b[k] == pytest.approx(a[k], rel=1e-12, abs=1e-15), (kw, k)
+
+
+# Masked lattices without empty sites (slabs): the one-site terms of the colour-0 sites a colour-1 site counts come
+# from their own point type and anchoring vector. Cases: the C3 physics (GHRL, homeotropic anchoring on both z
+# surfaces, field along x), LL with planar (rp) anchoring, a field with a polar term and the flip trial, GHRL with the
+# GHRL form of the homeotropic and planar anchorings, and a free x axis of odd length (last column of colour 0 counted
+# from its -x side).
+GHRL_K = (0.7272726, -1.6363635, -1.0, -0.1818182, 0.0)
+MASKED = {
+    "ghrl-slab-homeotropic-field": ((12, 10, 6), (1, 1, 0), ("ghrl", dict(ghrl=GHRL_K, elec_dir=(1.0, 0.0, 0.0),
+                                                                           elecA=0.25)),
+                                    [("homeotropic", 1.5, (0, 0, 0)), ("homeotropic", 1.5, (0, 0, 0))]),
+    "ll-slab-planar-flip": ((10, 8, 6), (1, 1, 0), ("ll", dict(elec_dir=(0.0, 0.6, 0.8), elecA=0.3, elecP=0.2)),
+                            [("rp", 1.0, (1.0, 0.0, 0.0)), ("rp", 0.7, (0.0, 1.0, 0.0))]),
+    "ghrl-slab-ghrl-anchorings": ((8, 8, 8), (1, 1, 0), ("ghrl", dict(ghrl=GHRL_K)),
+                                  [("homeotropic_ghrl", 1.2, (0, 0, 0)), ("rp_ghrl", 0.8, (1.0, 0.0, 0.0))]),
+    "ll-slab-free-odd-x": ((9, 8, 6), (0, 1, 0), ("ll", {}),
+                           [("homeotropic", 1.0, (0, 0, 0)), ("fg", 0.5, (0, 0, 0))]),
+}
+
+
+@pytest.mark.parametrize("name", list(MASKED))
+def test_fused_measurement_on_masked_slabs(mcl, name):
+    from mclics_amd import batch
+    dims, per, (pot, kw), surfaces = MASKED[name]
+    pt, ns = batch.slab_geometry(*dims)
+    lat = mcl.Lattice(*dims, rand_dirs(dims[0] * dims[1] * dims[2], 7), periodic=per, pt=pt, ns=ns,
+                      surfaces=surfaces, model=mcl.make_model(pot, **kw), T=0.8, seed=7)
+    with mcl.Engine(lat) as eng:
+        eng.sweeps(3)
+        for rep in range(2):
+            eng.reset_moments()
+            eng.sweeps(1, 1)
+            m = eng.moments()
+            o = eng.measure()
+            assert m["n"] == 1
+            assert m["E"] == pytest.approx(o["E"], rel=1e-6, abs=1e-7), name
+            assert m["S"] == pytest.approx(o["S"], rel=1e-10, abs=1e-12), name
