@@ -159,15 +159,13 @@ __device__ __forceinline__ float bond(const LatDev &L, V3 a, V3 b) {
   if constexpr (POT == MCL_POT_LL) {
     return nij * nij;
   } else if constexpr (POT == MCL_POT_GHRL) {
-    const float ai = SG * comp<AX>(a), aj = SG * comp<AX>(b);
+    // the achiral part; a_i = r.n_i and a_j = r.n_j enter only as a_i^2, a_j^2 and a_i a_j, so the bond sign drops
+    // out. The chiral term sigma (+-c) is bond_chiral, summed separately by bond_sum.
+    const float ai = comp<AX>(a), aj = comp<AX>(b);
     const float pij = 1.5f * nij * nij - 0.5f;
-    // c = r.(n_j x n_i) (potential.cpp:50-52)
-    float cr = AX == 0 ? fmaf(a.z, b.y, -(a.y * b.z))
-                       : (AX == 1 ? fmaf(a.x, b.z, -(a.z * b.x)) : fmaf(a.y, b.x, -(a.x * b.y)));
-    cr *= SG;
     const float E1 = 1.5f * fmaf(aj, aj, ai * ai) - 1.0f;
     const float e = fmaf(E1, fmaf(L.gr, pij, L.gl), L.gm * fmaf(ai * aj, nij, -(1.0f / 9.0f)));
-    return fmaf(L.gs, nij > 0.0f ? cr : -cr, fmaf(L.gn, pij, e));
+    return fmaf(L.gn, pij, e);
   } else {
     // pear: -A b^2 - B1 b - (C/4) (1 + b)^2 r.(n_j - n_i)
     const float dr = SG * (comp<AX>(b) - comp<AX>(a));
@@ -214,6 +212,16 @@ struct Site {
   uint32_t qx[2];  // NBX: pool indices of the +x and -x neighbours
 };
 
+// GHRL chiral factor of bond D: +-c with c = r.(n_j x n_i) (potential.cpp:50-52), the sign of n_i.n_j.
+template <int D>
+__device__ __forceinline__ float bond_chiral(V3 a, V3 b) {
+  constexpr int AX = D >> 1;
+  constexpr float SG = (D & 1) ? -1.0f : 1.0f;
+  const float cr = SG * (AX == 0 ? fmaf(a.z, b.y, -(a.y * b.z))
+                                 : (AX == 1 ? fmaf(a.x, b.z, -(a.z * b.x)) : fmaf(a.y, b.x, -(a.x * b.y))));
+  return dot3(a, b) > 0.0f ? cr : -cr;
+}
+
 // Bond part of the site energy: the bonds to the existing neighbours (LL scaled by -1.5 A).
 template <int POT>
 __device__ __forceinline__ float bond_sum(const LatDev &L, const Site &s, V3 n) {
@@ -233,6 +241,20 @@ __device__ __forceinline__ float bond_sum(const LatDev &L, const Site &s, V3 n) 
   e += (s.ex & 8u) ? bond<POT, 3>(L, n, s.nb[3]) : 0.0f;
   e += (s.ex & 16u) ? bond<POT, 4>(L, n, s.nb[4]) : 0.0f;
   e += (s.ex & 32u) ? bond<POT, 5>(L, n, s.nb[5]) : 0.0f;
+  if constexpr (POT == MCL_POT_GHRL) {
+    // sigma = 0 without chirality (setGHRL, io.cpp:340-360): a uniform branch skips the cross products. With
+    // sigma != 0 the chiral factors are summed and added as sigma x sum (fmaf(0, c, e) would be e exactly).
+    if (L.gs != 0.0f) {
+      float c = 0.0f;
+      c += (s.ex & 1u) ? bond_chiral<0>(n, s.nb[0]) : 0.0f;
+      c += (s.ex & 2u) ? bond_chiral<1>(n, s.nb[1]) : 0.0f;
+      c += (s.ex & 4u) ? bond_chiral<2>(n, s.nb[2]) : 0.0f;
+      c += (s.ex & 8u) ? bond_chiral<3>(n, s.nb[3]) : 0.0f;
+      c += (s.ex & 16u) ? bond_chiral<4>(n, s.nb[4]) : 0.0f;
+      c += (s.ex & 32u) ? bond_chiral<5>(n, s.nb[5]) : 0.0f;
+      e = fmaf(L.gs, c, e);
+    }
+  }
   return e;
 }
 
@@ -342,13 +364,17 @@ __device__ __forceinline__ bool gather(const LatDev &L, int colour, int t, const
   return true;
 }
 
-// gather<false, false, false> for a row-aligned lattice: every axis periodic, no mask, and nxh a multiple of 64
-// (nx a multiple of 128). The 64 positions [g, g + 64) of a lane group then lie in one row, so the row, j, k, the
-// parity of the row and the y / z wraps are wave-uniform (scalar registers): lane `lane` is position g + lane, and
-// only its x neighbours depend on the lane. Same sites, neighbours and order as gather().
-__device__ __forceinline__ void gather_row(const LatDev &L, int colour, uint32_t g, int lane,
+// gather() for a row-aligned lattice: nxh a multiple of 64 (nx a multiple of 128). The 64 positions [g, g + 64) of
+// a lane group then lie in one row, so the row, j, k, the parity of the row, the y / z wraps and the y / z free
+// faces are wave-uniform (scalar registers): lane `lane` is position g + lane, and only its x neighbours, the point
+// types and the anchoring vectors depend on the lane. Same sites, neighbours, existence mask and order as gather(),
+// missing neighbours included (they keep the site's own column of the other colour and are masked out).
+template <bool MASK, bool SURF, bool FREE, bool NBX>
+__device__ __forceinline__ bool gather_row(const LatDev &L, int colour, uint32_t g, int lane,
                                            const float *__restrict__ X, const float *__restrict__ Y,
-                                           const float *__restrict__ Z, Site &s, uint32_t &self) {
+                                           const float *__restrict__ Z, const uint8_t *__restrict__ PT,
+                                           const float *__restrict__ SX, const float *__restrict__ SY,
+                                           const float *__restrict__ SZ, Site &s, uint32_t &self) {
   const uint32_t nxh = (uint32_t)L.nxh;
   const uint32_t row = fdiv(g, L.mxh, L.sxh);
   const uint32_t m0 = g - row * nxh;
@@ -363,25 +389,81 @@ __device__ __forceinline__ void gather_row(const LatDev &L, int colour, uint32_t
   s.j = (int)j;
   s.k = (int)k;
   s.ptv = 1;
-  s.ex = 63u;
-  s.sv = V3{0.0f, 0.0f, 0.0f};
+  if constexpr (MASK) {
+    s.ptv = ldx(PT, self);
+    if (s.ptv == 0) return false;
+  }
   s.n = V3{ldx(X, self), ldx(Y, self), ldx(Z, self)};
-  // parity 0 (i = 2m): +x neighbour at m, -x at m - 1; parity 1 (i = 2m + 1): +x at m + 1, -x at m
+  const uint32_t col = ob + m;
+  unsigned ex = 63u;
+  // parity 0 (i = 2m): +x neighbour at m, -x at m - 1; parity 1 (i = 2m + 1): +x at m + 1, -x at m. At the row's
+  // ends the neighbour wraps (periodic x) or is missing (free x).
   uint32_t xp, xm;
   if (par) {
-    xp = ob + (m + 1 == nxh ? 0u : m + 1);
-    xm = ob + m;
+    const bool edge = m + 1 == nxh;
+    xp = edge ? ob : col + 1;
+    xm = col;
+    if constexpr (FREE) {
+      if (!(L.flags & F_PX) && edge) {
+        xp = col;
+        ex &= ~1u;
+      }
+    }
   } else {
-    xp = ob + m;
-    xm = ob + (m == 0 ? nxh - 1 : m - 1);
+    const bool edge = m == 0;
+    xp = col;
+    xm = edge ? ob + nxh - 1 : col - 1;
+    if constexpr (FREE) {
+      if (!(L.flags & F_PX) && edge) {
+        xm = col;
+        ex &= ~2u;
+      }
+    }
   }
-  const uint32_t yp = (j + 1 == (uint32_t)L.ny ? ob - L.ywrap : ob + nxh) + m;
-  const uint32_t ym = (j == 0 ? ob + L.ywrap : ob - nxh) + m;
-  const uint32_t zp = (k + 1 == (uint32_t)L.nz ? ob - L.zwrap : ob + L.plane) + m;
-  const uint32_t zm = (k == 0 ? ob + L.zwrap : ob - L.plane) + m;
+  uint32_t yp = (j + 1 == (uint32_t)L.ny ? ob - L.ywrap : ob + nxh) + m;
+  uint32_t ym = (j == 0 ? ob + L.ywrap : ob - nxh) + m;
+  uint32_t zp = (k + 1 == (uint32_t)L.nz ? ob - L.zwrap : ob + L.plane) + m;
+  uint32_t zm = (k == 0 ? ob + L.zwrap : ob - L.plane) + m;
+  if constexpr (FREE) {
+    if (!(L.flags & F_PY)) {
+      if (j + 1 == (uint32_t)L.ny) {
+        yp = col;
+        ex &= ~4u;
+      }
+      if (j == 0) {
+        ym = col;
+        ex &= ~8u;
+      }
+    }
+    if (!(L.flags & F_PZ)) {
+      if (k + 1 == (uint32_t)L.nz) {
+        zp = col;
+        ex &= ~16u;
+      }
+      if (k == 0) {
+        zm = col;
+        ex &= ~32u;
+      }
+    }
+  }
   const uint32_t nidx[6] = {xp, xm, yp, ym, zp, zm};
 #pragma unroll
-  for (int d = 0; d < 6; d++) s.nb[d] = V3{ldx(X, nidx[d]), ldx(Y, nidx[d]), ldx(Z, nidx[d])};
+  for (int d = 0; d < 6; d++) {
+    const uint32_t q = nidx[d];
+    s.nb[d] = V3{ldx(X, q), ldx(Y, q), ldx(Z, q)};
+    if constexpr (MASK) {
+      if (ldx(PT, q) == 0) ex &= ~(1u << d);
+      if constexpr (NBX) {
+        if (d < 2) s.qx[d] = q;
+      }
+    }
+  }
+  s.ex = ex;
+  s.sv = V3{0.0f, 0.0f, 0.0f};
+  if constexpr (SURF) {
+    if (s.ptv > 1) s.sv = V3{ldx(SX, self), ldx(SY, self), ldx(SZ, self)};
+  }
+  return true;
 }
 
 // The site alone (no neighbours): decodes position t, loads its point type, director and anchoring vector.
@@ -527,9 +609,9 @@ template <int POT, bool MASK, bool SURF, bool FIELD, bool FLIP, bool FREE, bool 
 // energy minus their one-site terms, plus the one-site terms of colour 0. Each colour-0 site is counted (Q tensor,
 // one-site terms, count) by the colour-1 site on its +x side, or, at the last column of a free x axis, by the one on
 // its -x side. The workgroup writes kNQS FP64 partials to spart[q * nblk + workgroup].
-// ROW: every lattice of the context is row-aligned (gather_row): site decoding and the y / z neighbour indices
-// are scalar work.
-__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(MEAS && MASK ? 6 : 8, 8))) void k_sweep(
+// ROW: every lattice of the context is row-aligned (gather_row): site decoding, the y / z neighbour indices and
+// the y / z free faces are scalar work.
+__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(MEAS && (MASK || POT != MCL_POT_LL) ? 6 : 8, 8))) void k_sweep(
     const LatDev *__restrict__ lats, CtrlDev *__restrict__ ctrl,
                                                 unsigned int *__restrict__ cnt, const int2 *__restrict__ blk,
                                                 int nblk, int chunk, int xcd, int colour,
@@ -541,7 +623,6 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(MEAS && MA
                                                   double *__restrict__ spart, int fin, int nlat,
                                                   const int2 *__restrict__ srange, MomDev *__restrict__ mom,
                                                   ObsDev *__restrict__ obs) {
-  static_assert(!(ROW && (MASK || SURF || FREE)), "row-aligned gathering needs unmasked all-periodic lattices");
   const int b = blockIdx.x;
   // logical workgroups [0, nfin) finalize the measurement folded into the previous sweep (fin = 1), one per lattice:
   // they come first in dispatch order, so their FP64 eigen-solve runs beside the sweep tiles instead of after them.
@@ -614,8 +695,9 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(MEAS && MA
       // first position of this lane group, wave-uniform (a wave covers 64 SPT consecutive positions)
       const uint32_t g = (uint32_t)bd.y + (uint32_t)(64 * SPT * __builtin_amdgcn_readfirstlane(threadIdx.x >> 6)) +
                          (uint32_t)(64 * e);
-      if (g >= (uint32_t)L.nc) continue;
-      gather_row(L, colour, g, lane, X, Y, Z, s, self);
+      if (g >= (uint32_t)L.nc ||
+          !gather_row<MASK, SURF, FREE, MEAS && MASK>(L, colour, g, lane, X, Y, Z, PT, SX, SY, SZ, s, self))
+        continue;
     } else {
       const int t = tw + 64 * e;
       if (t >= L.nc || !gather<MASK, SURF, FREE, MEAS && MASK>(L, colour, t, X, Y, Z, PT, SX, SY, SZ, s, self))
@@ -1137,7 +1219,7 @@ struct Ctx {
   int pot = 0;
   bool mask = false, surf = false, field = false, flip = false;
   bool free = false;  // some lattice has a free (non-periodic) axis
-  bool row = false;   // every lattice is row-aligned: all-periodic, unmasked, nx a multiple of 128 (gather_row)
+  bool row = false;   // every lattice is row-aligned: nx a multiple of 128 (gather_row)
   bool xq = false;  // radial or pear observables are measured
   std::vector<LatHost> lh;
   std::vector<LatDev> ld;
@@ -1187,14 +1269,12 @@ void launch_sweep_k(dim3 g, dim3 b, hipStream_t st, const Ctx &c, int colour, un
 
 template <int POT, bool MASK, bool SURF, bool FIELD, bool FLIP, bool FREE>
 void launch_sweep(dim3 g, dim3 b, hipStream_t st, const Ctx &c, int colour, unsigned k, bool meas, int fin) {
-  if constexpr (!MASK && !SURF && !FREE) {
-    if (c.row) {
-      if (meas)
-        launch_sweep_k<POT, MASK, SURF, FIELD, FLIP, FREE, true, true>(g, b, st, c, colour, k, fin);
-      else
-        launch_sweep_k<POT, MASK, SURF, FIELD, FLIP, FREE, false, true>(g, b, st, c, colour, k, fin);
-      return;
-    }
+  if (c.row) {
+    if (meas)
+      launch_sweep_k<POT, MASK, SURF, FIELD, FLIP, FREE, true, true>(g, b, st, c, colour, k, fin);
+    else
+      launch_sweep_k<POT, MASK, SURF, FIELD, FLIP, FREE, false, true>(g, b, st, c, colour, k, fin);
+    return;
   }
   if (meas) {
     launch_sweep_k<POT, MASK, SURF, FIELD, FLIP, FREE, true, false>(g, b, st, c, colour, k, fin);
@@ -1510,7 +1590,7 @@ int create_ctx(const mcl_lattice_desc *lat, int nlat, Ctx *c) {
   // the measurement is folded into the sweep unless radial or pear observables are measured, some lattice is a
   // single column in x, or some lattice has empty sites (each colour-0 site is counted by an x neighbour)
   c->fused = !c->xq;
-  c->row = !c->mask && !c->surf && !c->free;
+  c->row = true;
   for (int r = 0; r < nlat; r++) {
     if (c->ld[r].nx < 2) c->fused = false;
     if ((long long)c->ld[r].nvalid != (long long)c->ld[r].nx * c->ld[r].ny * c->ld[r].nz) c->fused = false;
