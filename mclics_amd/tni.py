@@ -57,8 +57,10 @@ def coexistence_point(T, drift, half=3):
 
     The split between neighbouring points with the fewest points on the wrong side is located first (ties: the
     middle of the tied splits); a straight line fitted to the `half` points on each side of it then gives the zero.
-    Returns (T0, bracketed, side): bracketed is False when every drift has one sign, and side is then +1 when T_NI
-    lies above the window (all drifts positive) and -1 when it lies below."""
+    A split with fewer than two points on one side is no bracket: one noisy point at the window's edge does not
+    make a sign change. Returns (T0, bracketed, side): bracketed is False when every drift has one sign or the
+    split lies at an edge, and side is then +1 when T_NI lies above the window (drifts positive) and -1 when it
+    lies below."""
     T = np.asarray(T, dtype=np.float64)
     d = np.asarray(drift, dtype=np.float64)
     order = np.argsort(T)
@@ -71,6 +73,10 @@ def coexistence_point(T, drift, half=3):
     cost = np.array([np.sum(d[:j + 1] < 0) + np.sum(d[j + 1:] > 0) for j in range(n - 1)])
     ties = np.flatnonzero(cost == cost.min())
     j = int(ties[len(ties) // 2])
+    if j + 1 < 2:  # at most one point below the split: T_NI lies below the window
+        return float(T[0]), False, -1
+    if n - (j + 1) < 2:  # at most one point above it: T_NI lies above the window
+        return float(T[-1]), False, +1
     split = 0.5 * (T[j] + T[j + 1])
     lo, hi = max(0, j + 1 - half), min(n, j + 1 + half)
     b, a = np.polyfit(T[lo:hi], d[lo:hi], 1)

@@ -55,6 +55,18 @@ def test_coexistence_point_outside_the_window():
     T = np.linspace(1.0, 1.05, 20)
     assert tni.coexistence_point(T, np.full(20, 0.01)) == (pytest.approx(1.05), False, 1)
     assert tni.coexistence_point(T, np.full(20, -0.01)) == (pytest.approx(1.0), False, -1)
+    # one noisy point of the other sign at an edge is no bracket: the window moves towards the majority
+    d = np.full(20, 0.01)
+    d[-1] = -0.002
+    assert tni.coexistence_point(T, d) == (pytest.approx(1.05), False, 1)
+    d = np.full(20, -0.01)
+    d[0] = 0.003
+    assert tni.coexistence_point(T, d) == (pytest.approx(1.0), False, -1)
+    # two points beyond the split are a bracket
+    d = np.full(20, 0.01)
+    d[-2:] = -0.01
+    est, ok, side = tni.coexistence_point(T, d)
+    assert ok and side == 0 and T[-3] <= est <= T[-2]
 
 
 @pytest.mark.parametrize("S", [0.0, 0.15, 0.33, 0.6, 0.9])
