@@ -1,11 +1,11 @@
-"""Writes tests/golden/c1_oracle.npz: the oracle's thermal run of the 16^3 LL case, 32 replicas.
+"""Writes tests/golden/c1_oracle.npz: the oracle's thermal run of the 16^3 LL case, 64 replicas.
 
 Case (the CPU configuration of the benchmark): Lebwohl-Lasher A = 1, 16x16x16 bulk, periodic x/y/z, random
 initial directors from the replica's seed, T = 1.2 -> 0.1 with dT = -0.02 (56 points), MCT = 500, MCS = 200.
 Replica r uses seed r + 1 for both the initial condition and the sweep RNG. One process per replica, one thread
 each, so the fixture does not depend on the machine's core count.
 
-    python tests/golden/make_c1_fixture.py [--reps 32] [--out tests/golden/c1_oracle.npz]
+    python tests/golden/make_c1_fixture.py [--reps 64] [--out tests/golden/c1_oracle.npz]
 """
 import argparse
 import multiprocessing as mp
@@ -39,7 +39,7 @@ def one(seed):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--reps", type=int, default=32)
+    ap.add_argument("--reps", type=int, default=64)
     ap.add_argument("--out", default=os.path.join(HERE, "c1_oracle.npz"))
     ap.add_argument("--workers", type=int, default=min(8, os.cpu_count() or 1))
     a = ap.parse_args()
