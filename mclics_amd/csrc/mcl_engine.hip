@@ -288,7 +288,9 @@ __device__ __forceinline__ float site_energy(const LatDev &L, const Site &s, V3 
 // exists and the wrap is the only boundary case.
 // NBX (measuring kernels of masked lattices): also keeps the pool indices of the two x neighbours, whose one-site
 // terms the measurement adds for the colour-0 sites a colour-1 site counts.
-template <bool MASK, bool SURF, bool FREE = true, bool NBX = false>
+// NBPT = false (the LL sweep): the neighbours' point types are not loaded. An LL context stores its empty sites as
+// zero vectors, whose bonds add fmaf(0, 0, e) = e to the LL sum, exactly what the existence mask would give.
+template <bool MASK, bool SURF, bool FREE = true, bool NBX = false, bool NBPT = true>
 __device__ __forceinline__ bool gather(const LatDev &L, int colour, int t, const float *__restrict__ X,
                                        const float *__restrict__ Y, const float *__restrict__ Z,
                                        const uint8_t *__restrict__ PT, const float *__restrict__ SX,
@@ -350,7 +352,9 @@ __device__ __forceinline__ bool gather(const LatDev &L, int colour, int t, const
     const uint32_t q = nidx[d];
     s.nb[d] = V3{ldx(X, q), ldx(Y, q), ldx(Z, q)};
     if constexpr (MASK) {
-      if (ldx(PT, q) == 0) ex &= ~(1u << d);
+      if constexpr (NBPT) {
+        if (ldx(PT, q) == 0) ex &= ~(1u << d);
+      }
       if constexpr (NBX) {
         if (d < 2) s.qx[d] = q;
       }
@@ -369,7 +373,7 @@ __device__ __forceinline__ bool gather(const LatDev &L, int colour, int t, const
 // faces are wave-uniform (scalar registers): lane `lane` is position g + lane, and only its x neighbours, the point
 // types and the anchoring vectors depend on the lane. Same sites, neighbours, existence mask and order as gather(),
 // missing neighbours included (they keep the site's own column of the other colour and are masked out).
-template <bool MASK, bool SURF, bool FREE, bool NBX>
+template <bool MASK, bool SURF, bool FREE, bool NBX, bool NBPT = true>
 __device__ __forceinline__ bool gather_row(const LatDev &L, int colour, uint32_t g, int lane,
                                            const float *__restrict__ X, const float *__restrict__ Y,
                                            const float *__restrict__ Z, const uint8_t *__restrict__ PT,
@@ -452,7 +456,9 @@ __device__ __forceinline__ bool gather_row(const LatDev &L, int colour, uint32_t
     const uint32_t q = nidx[d];
     s.nb[d] = V3{ldx(X, q), ldx(Y, q), ldx(Z, q)};
     if constexpr (MASK) {
-      if (ldx(PT, q) == 0) ex &= ~(1u << d);
+      if constexpr (NBPT) {
+        if (ldx(PT, q) == 0) ex &= ~(1u << d);
+      }
       if constexpr (NBX) {
         if (d < 2) s.qx[d] = q;
       }
@@ -696,11 +702,13 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(MEAS && (M
       const uint32_t g = (uint32_t)bd.y + (uint32_t)(64 * SPT * __builtin_amdgcn_readfirstlane(threadIdx.x >> 6)) +
                          (uint32_t)(64 * e);
       if (g >= (uint32_t)L.nc ||
-          !gather_row<MASK, SURF, FREE, MEAS && MASK>(L, colour, g, lane, X, Y, Z, PT, SX, SY, SZ, s, self))
+          !gather_row<MASK, SURF, FREE, MEAS && MASK, POT != MCL_POT_LL>(L, colour, g, lane, X, Y, Z, PT, SX, SY,
+                                                                          SZ, s, self))
         continue;
     } else {
       const int t = tw + 64 * e;
-      if (t >= L.nc || !gather<MASK, SURF, FREE, MEAS && MASK>(L, colour, t, X, Y, Z, PT, SX, SY, SZ, s, self))
+      if (t >= L.nc || !gather<MASK, SURF, FREE, MEAS && MASK, POT != MCL_POT_LL>(L, colour, t, X, Y, Z, PT, SX, SY,
+                                                                                 SZ, s, self))
         continue;
     }
     float u_ang, rot, u;
@@ -1210,6 +1218,10 @@ struct LatHost {
   int nvalid;
   bool mask, surf;
   int radial;
+  // LL context (Ctx::zero_empty): the empty sites (linear index) are zero vectors on the device; the directors the
+  // caller gave them are kept here and returned by mcl_get_directors
+  std::vector<uint32_t> empty;
+  std::vector<float> empty_ni;
 };
 
 struct Ctx {
@@ -1220,6 +1232,7 @@ struct Ctx {
   bool mask = false, surf = false, field = false, flip = false;
   bool free = false;  // some lattice has a free (non-periodic) axis
   bool row = false;   // every lattice is row-aligned: nx a multiple of 128 (gather_row)
+  bool zero_empty = false;  // LL context with masks: empty sites are stored as zero vectors (see gather's NBPT)
   bool xq = false;  // radial or pear observables are measured
   std::vector<LatHost> lh;
   std::vector<LatDev> ld;
@@ -1366,9 +1379,21 @@ void aos_to_soa(const LatHost &h, const float *ni, std::vector<float> &x, std::v
 }
 
 int upload_directors(Ctx *c, int r, const float *ni) {
-  const LatHost &h = c->lh[r];
+  LatHost &h = c->lh[r];
   std::vector<float> x, y, z;
   aos_to_soa(h, ni, x, y, z);
+  if (c->zero_empty && !h.empty.empty()) {
+    // keep the caller's directors of the empty sites, store zero vectors
+    h.empty_ni.resize(3 * h.empty.size());
+    for (size_t e = 0; e < h.empty.size(); e++) {
+      const uint32_t lin = h.empty[e];
+      for (int a = 0; a < 3; a++) h.empty_ni[3 * e + a] = ni[3 * (size_t)lin + a];
+      const int i = (int)(lin % (uint32_t)h.nx), j = (int)((lin / (uint32_t)h.nx) % (uint32_t)h.ny);
+      const int k = (int)(lin / ((uint32_t)h.nx * (uint32_t)h.ny));
+      const size_t t = (size_t)((i + j + k) & 1) * h.nc + (i >> 1) + (size_t)h.nxh * (j + (size_t)h.ny * k);
+      x[t] = y[t] = z[t] = 0.0f;
+    }
+  }
   const size_t bytes = sizeof(float) * 2 * (size_t)h.nc;
   MCL_HIP(hipMemcpyAsync(c->X + h.off, x.data(), bytes, hipMemcpyHostToDevice, c->stream));
   MCL_HIP(hipMemcpyAsync(c->Y + h.off, y.data(), bytes, hipMemcpyHostToDevice, c->stream));
@@ -1591,6 +1616,7 @@ int create_ctx(const mcl_lattice_desc *lat, int nlat, Ctx *c) {
   // single column in x, or some lattice has empty sites (each colour-0 site is counted by an x neighbour)
   c->fused = !c->xq;
   c->row = true;
+  c->zero_empty = c->pot == MCL_POT_LL && c->mask;
   for (int r = 0; r < nlat; r++) {
     if (c->ld[r].nx < 2) c->fused = false;
     if ((long long)c->ld[r].nvalid != (long long)c->ld[r].nx * c->ld[r].ny * c->ld[r].nz) c->fused = false;
@@ -1675,6 +1701,12 @@ int create_ctx(const mcl_lattice_desc *lat, int nlat, Ctx *c) {
       MCL_HIP(hipMemcpy(c->SX + h.off, x.data(), bytes, hipMemcpyHostToDevice));
       MCL_HIP(hipMemcpy(c->SY + h.off, y.data(), bytes, hipMemcpyHostToDevice));
       MCL_HIP(hipMemcpy(c->SZ + h.off, z.data(), bytes, hipMemcpyHostToDevice));
+    }
+    if (c->zero_empty && d.pt && h.nvalid < (long long)h.nx * h.ny * h.nz) {
+      LatHost &hw = c->lh[r];
+      const long long Nt = (long long)h.nx * h.ny * h.nz;
+      for (long long q = 0; q < Nt; q++)
+        if (d.pt[q] == 0) hw.empty.push_back((uint32_t)q);
     }
     if ((st = upload_directors(c, r, d.ni))) return st;
   }
@@ -2075,6 +2107,8 @@ int mcl_get_directors(void *ctx, int32_t lat, float *ni) {
         ni[3 * lin + 1] = y[t];
         ni[3 * lin + 2] = z[t];
       }
+  for (size_t e = 0; e < h.empty.size() && c->zero_empty; e++)
+    for (int a = 0; a < 3; a++) ni[3 * (size_t)h.empty[e] + a] = h.empty_ni[3 * e + a];
   return MCL_OK;
 }
 

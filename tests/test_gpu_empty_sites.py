@@ -55,3 +55,27 @@ def test_list_sweep_is_launch_shape_and_batch_independent(mcl):
         d, m = run(shape, extra)
         assert np.array_equal(d, ref_d), (shape, extra)
         assert m == ref_m
+
+
+def test_set_directors_mid_run_keeps_the_empty_sites(mcl):
+    """An LL context keeps its empty sites as zero vectors on the device (their bonds add nothing to the LL sum) and
+    the caller's directors for them on the host: mcl_set_directors in the middle of a run must return them unchanged
+    and must not change the sweep of the non-empty sites."""
+    lat, pt, ni0 = holes_lattice(mcl, (18, 16, 14), 0.3, 3, 1.0)
+    rng = np.random.default_rng(4)
+    with mcl.Engine(lat) as eng, mcl.Engine(holes_lattice(mcl, (18, 16, 14), 0.3, 3, 1.0)[0]) as twin:
+        eng.sweeps(10, 5)
+        twin.sweeps(10, 5)
+        d = eng.directors().reshape(-1, 3)
+        empty = pt == 0
+        new_empty = rng.normal(size=(int(empty.sum()), 3)).astype(np.float32)
+        d[empty] = new_empty
+        eng.set_directors(d.reshape(-1))
+        got = eng.directors().reshape(-1, 3)
+        assert np.array_equal(got[empty], new_empty)
+        eng.sweeps(10, 5)
+        twin.sweeps(10, 5)
+        a, b = eng.directors().reshape(-1, 3), twin.directors().reshape(-1, 3)
+        assert np.array_equal(a[~empty], b[~empty])
+        assert np.array_equal(a[empty], new_empty)
+        assert eng.moments() == twin.moments()
