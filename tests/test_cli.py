@@ -311,3 +311,49 @@ def test_snapshot_format(run):
     assert lines[2].split(",")[:3] == ["1", "0", "0"]
     mid = lines[1 + 3 + 7 * (3 + 7 * 3)].split(",")
     assert mid[:3] == ["3", "3", "3"] and mid[7] == "1" and 0 < float(mid[6]) <= 1.0
+
+
+def read_ic(r, name="ic.csv"):
+    with open(os.path.join(r.dir, name)) as f:
+        assert next(f).strip() == "x,y,z,nx,ny,nz,S,pt"
+        return [l.strip().split(",") for l in f if l.strip()]
+
+
+@pytestmark_gpu
+def test_cholesteric_ic(run):
+    """ic cholesteric (ic.cpp:202-225): on every non-empty site phi = phi_0 - 2 pi k / p0 and n = (sin theta_0 cos phi,
+    sin theta_0 sin phi, cos theta_0), with the reference's float / double steps; p0_i replaces p0 when set."""
+    import numpy as np
+    f32 = np.float32
+    for extra, pitch in (((), 8.0), (("p0_i 5",), 5.0)):
+        r = run(plus(base(Nx=4, Ny=2, Nz=12, ic="cholesteric", theta_0=70, phi_0=30), "p0 8", *extra)).ok()
+        rows = read_ic(r)
+        assert len(rows) == 4 * 2 * 12
+        phi_0 = f32(30 * math.pi / 180)
+        st, ct = math.sin(f32(70) * math.pi / 180), math.cos(f32(70) * math.pi / 180)
+        for row in rows:
+            k = int(row[2])
+            phi = float(f32(phi_0 - 2 * math.pi * k / f32(pitch)))
+            want = (f32(st * math.cos(phi)), f32(st * math.sin(phi)), f32(ct))
+            for a in range(3):
+                close("n%d at k=%d" % (a, k), float(row[3 + a]), float(want[a]), 2e-6)
+
+
+@pytestmark_gpu
+def test_custom_geometry_ground_state(run):
+    """geometry custom with a boundary file (geometry_custom.cpp:70-163): plane k = 0 empty, plane k = 3 a homeotropic
+    surface with normal z (pt 2), the rest unlisted and therefore bulk. Periodic 4 x 4 x 6, uniform along z at T = 0:
+    the sites next to the empty plane have 5 bonds (-7.5), the surface plane adds -W, so
+    E = (2 (-7.5) + 2 (-9) + (-9 - W)) / 5 per site with W = 2, and S = 1."""
+    lines = ["x,y,z,nx,ny,nz,pt"]
+    for j in range(4):
+        for i in range(4):
+            lines.append("%d,%d,0,0,0,1,0" % (i, j))
+            lines.append("%d,%d,3,0,0,1,2" % (i, j))
+    r = run(plus(base(Nx=4, Ny=4, Nz=6, geometry="custom"), "boundary_file geom.csv", "anchoring_type 0 homeotropic",
+                 "W 0 2"), files={"geom.csv": "\n".join(lines) + "\n"}).ok()
+    _, rows = r.po()
+    close("E", rows[0]["E"], (2 * -7.5 + 2 * -9 + (-9 - 2)) / 5, 1e-4)
+    close("S", rows[0]["S"], 1.0, 1e-5)
+    pts = {int(row[2]): int(row[7]) for row in read_ic(r)}
+    assert pts == {0: 0, 1: 1, 2: 1, 3: 2, 4: 1, 5: 1}
