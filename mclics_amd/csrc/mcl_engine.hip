@@ -235,23 +235,32 @@ __device__ __forceinline__ float bond_sum(const LatDev &L, const Site &s, V3 n) 
     }
     return e * (-1.5f * L.A);
   }
-  e += (s.ex & 1u) ? bond<POT, 0>(L, n, s.nb[0]) : 0.0f;
-  e += (s.ex & 2u) ? bond<POT, 1>(L, n, s.nb[1]) : 0.0f;
-  e += (s.ex & 4u) ? bond<POT, 2>(L, n, s.nb[2]) : 0.0f;
-  e += (s.ex & 8u) ? bond<POT, 3>(L, n, s.nb[3]) : 0.0f;
-  e += (s.ex & 16u) ? bond<POT, 4>(L, n, s.nb[4]) : 0.0f;
-  e += (s.ex & 32u) ? bond<POT, 5>(L, n, s.nb[5]) : 0.0f;
+  // Every bond is evaluated and then selected: with the bond expression inside the conditional, the compiler
+  // branches around each one under an exec mask, which costs more than the bond itself when (as in the bulk of a
+  // slab) every neighbour exists. A missing neighbour adds +0.0f either way, so the sums are the same.
+  const float b0 = bond<POT, 0>(L, n, s.nb[0]), b1 = bond<POT, 1>(L, n, s.nb[1]);
+  const float b2 = bond<POT, 2>(L, n, s.nb[2]), b3 = bond<POT, 3>(L, n, s.nb[3]);
+  const float b4 = bond<POT, 4>(L, n, s.nb[4]), b5 = bond<POT, 5>(L, n, s.nb[5]);
+  e += (s.ex & 1u) ? b0 : 0.0f;
+  e += (s.ex & 2u) ? b1 : 0.0f;
+  e += (s.ex & 4u) ? b2 : 0.0f;
+  e += (s.ex & 8u) ? b3 : 0.0f;
+  e += (s.ex & 16u) ? b4 : 0.0f;
+  e += (s.ex & 32u) ? b5 : 0.0f;
   if constexpr (POT == MCL_POT_GHRL) {
     // sigma = 0 without chirality (setGHRL, io.cpp:340-360): a uniform branch skips the cross products. With
     // sigma != 0 the chiral factors are summed and added as sigma x sum (fmaf(0, c, e) would be e exactly).
     if (L.gs != 0.0f) {
+      const float c0 = bond_chiral<0>(n, s.nb[0]), c1 = bond_chiral<1>(n, s.nb[1]);
+      const float c2 = bond_chiral<2>(n, s.nb[2]), c3 = bond_chiral<3>(n, s.nb[3]);
+      const float c4 = bond_chiral<4>(n, s.nb[4]), c5 = bond_chiral<5>(n, s.nb[5]);
       float c = 0.0f;
-      c += (s.ex & 1u) ? bond_chiral<0>(n, s.nb[0]) : 0.0f;
-      c += (s.ex & 2u) ? bond_chiral<1>(n, s.nb[1]) : 0.0f;
-      c += (s.ex & 4u) ? bond_chiral<2>(n, s.nb[2]) : 0.0f;
-      c += (s.ex & 8u) ? bond_chiral<3>(n, s.nb[3]) : 0.0f;
-      c += (s.ex & 16u) ? bond_chiral<4>(n, s.nb[4]) : 0.0f;
-      c += (s.ex & 32u) ? bond_chiral<5>(n, s.nb[5]) : 0.0f;
+      c += (s.ex & 1u) ? c0 : 0.0f;
+      c += (s.ex & 2u) ? c1 : 0.0f;
+      c += (s.ex & 4u) ? c2 : 0.0f;
+      c += (s.ex & 8u) ? c3 : 0.0f;
+      c += (s.ex & 16u) ? c4 : 0.0f;
+      c += (s.ex & 32u) ? c5 : 0.0f;
       e = fmaf(L.gs, c, e);
     }
   }
