@@ -235,18 +235,23 @@ __device__ __forceinline__ float bond_sum(const LatDev &L, const Site &s, V3 n) 
     }
     return e * (-1.5f * L.A);
   }
-  // Every bond is evaluated and then selected: with the bond expression inside the conditional, the compiler
-  // branches around each one under an exec mask, which costs more than the bond itself when (as in the bulk of a
-  // slab) every neighbour exists. A missing neighbour adds +0.0f either way, so the sums are the same.
+  // Every bond is evaluated and then summed in direction order: a plain sum when all six neighbours exist (the
+  // bulk of any lattice), and a sum of selected terms otherwise, where a missing neighbour adds +0.0f. Both add the
+  // same values in the same order. (With the bond expressions inside the conditionals the compiler branches around
+  // each one under an exec mask, and per-bond selects on every site cost a quarter of the bond arithmetic.)
   const float b0 = bond<POT, 0>(L, n, s.nb[0]), b1 = bond<POT, 1>(L, n, s.nb[1]);
   const float b2 = bond<POT, 2>(L, n, s.nb[2]), b3 = bond<POT, 3>(L, n, s.nb[3]);
   const float b4 = bond<POT, 4>(L, n, s.nb[4]), b5 = bond<POT, 5>(L, n, s.nb[5]);
-  e += (s.ex & 1u) ? b0 : 0.0f;
-  e += (s.ex & 2u) ? b1 : 0.0f;
-  e += (s.ex & 4u) ? b2 : 0.0f;
-  e += (s.ex & 8u) ? b3 : 0.0f;
-  e += (s.ex & 16u) ? b4 : 0.0f;
-  e += (s.ex & 32u) ? b5 : 0.0f;
+  if (s.ex == 63u) {
+    e = ((((b0 + b1) + b2) + b3) + b4) + b5;
+  } else {
+    e += (s.ex & 1u) ? b0 : 0.0f;
+    e += (s.ex & 2u) ? b1 : 0.0f;
+    e += (s.ex & 4u) ? b2 : 0.0f;
+    e += (s.ex & 8u) ? b3 : 0.0f;
+    e += (s.ex & 16u) ? b4 : 0.0f;
+    e += (s.ex & 32u) ? b5 : 0.0f;
+  }
   if constexpr (POT == MCL_POT_GHRL) {
     // sigma = 0 without chirality (setGHRL, io.cpp:340-360): a uniform branch skips the cross products. With
     // sigma != 0 the chiral factors are summed and added as sigma x sum (fmaf(0, c, e) would be e exactly).
@@ -255,12 +260,16 @@ __device__ __forceinline__ float bond_sum(const LatDev &L, const Site &s, V3 n) 
       const float c2 = bond_chiral<2>(n, s.nb[2]), c3 = bond_chiral<3>(n, s.nb[3]);
       const float c4 = bond_chiral<4>(n, s.nb[4]), c5 = bond_chiral<5>(n, s.nb[5]);
       float c = 0.0f;
-      c += (s.ex & 1u) ? c0 : 0.0f;
-      c += (s.ex & 2u) ? c1 : 0.0f;
-      c += (s.ex & 4u) ? c2 : 0.0f;
-      c += (s.ex & 8u) ? c3 : 0.0f;
-      c += (s.ex & 16u) ? c4 : 0.0f;
-      c += (s.ex & 32u) ? c5 : 0.0f;
+      if (s.ex == 63u) {
+        c = ((((c0 + c1) + c2) + c3) + c4) + c5;
+      } else {
+        c += (s.ex & 1u) ? c0 : 0.0f;
+        c += (s.ex & 2u) ? c1 : 0.0f;
+        c += (s.ex & 4u) ? c2 : 0.0f;
+        c += (s.ex & 8u) ? c3 : 0.0f;
+        c += (s.ex & 16u) ? c4 : 0.0f;
+        c += (s.ex & 32u) ? c5 : 0.0f;
+      }
       e = fmaf(L.gs, c, e);
     }
   }
