@@ -19,6 +19,7 @@ d = Nz - 1 (make_figures.py:85-105), which has no free parameter.
 
     python -m mclics_amd.freedericksz run --out runs            # the reference's protocol, 2 seeds, all cells at once
     python -m mclics_amd.freedericksz analyze runs --json out.json
+    python -m mclics_amd.freedericksz run --batched --json out.json   # the same runs in one batched context
 """
 import argparse
 import json
@@ -90,6 +91,26 @@ def snapshot_scale(step):
     return s
 
 
+def sweep_points(a, b, d):
+    """Number of points of a sweep from a to b in steps of d (include/sweep.h:13-34)."""
+    return int(math.floor((b - a) / d + 1e-3)) + 1
+
+
+def sweep_value(a, b, d, n, npoints):
+    """Point n of that sweep; the last point is b exactly when the steps reach it."""
+    if n == npoints - 1 and abs((b - a) / d - (npoints - 1)) < 1e-3:
+        return b
+    return a + n * d
+
+
+def field_values(Ei, Ef, dE):
+    """The FP32 field values of an electric run: the FP32 inputs, the schedule in double, each value rounded to FP32."""
+    f = np.float32
+    a, b, d = float(f(Ei)), float(f(Ef)), float(f(dE))
+    n = sweep_points(a, b, d)
+    return [float(f(sweep_value(a, b, d, i, n))) for i in range(n)]
+
+
 # ---------------------------------------------------------------------------------------------------------------
 # Analysis
 
@@ -152,23 +173,20 @@ def continuum_fields(k11, k22, k33, Nz, W, deps):
     return out
 
 
-def analyze(root):
-    """Critical fields and elastic-constant ratios of a campaign directory root/<cell>/seed<s>/ (finished runs)."""
+def analyze_curves(curves, k11, k22, k33, Nz, W, deps):
+    """Critical fields and elastic-constant ratios from {cell: [(fields, angles) per seed]}, and the continuum
+    theory for the same slab."""
     res = {"cells": {}}
-    for cell, axis in CELL_AXIS.items():
-        runs = sorted(os.path.join(root, cell, d) for d in os.listdir(os.path.join(root, cell))
-                      if os.path.exists(os.path.join(root, cell, d, "DONE"))) \
-            if os.path.isdir(os.path.join(root, cell)) else []
-        if not runs:
+    for cell in CELL_AXIS:
+        if not curves.get(cell):
             continue
-        curves = [angle_curve(r, axis) for r in runs]
-        n = min(len(c[0]) for c in curves)
-        E = curves[0][0][:n]
-        th = np.array([c[1][:n] for c in curves])
+        n = min(len(c[0]) for c in curves[cell])
+        E = np.asarray(curves[cell][0][0][:n], dtype=np.float64)
+        th = np.array([c[1][:n] for c in curves[cell]], dtype=np.float64)
         ec, e20 = critical_field(E, th.mean(axis=0))
         per_seed = [x for x in (critical_field(E, t)[0] for t in th) if x is not None]
         err = float(np.std(per_seed, ddof=1) / math.sqrt(len(per_seed))) if len(per_seed) > 1 else None
-        res["cells"][cell] = dict(seeds=len(runs), E_c=ec, E_c_err=err, E20=e20, per_seed=per_seed,
+        res["cells"][cell] = dict(seeds=len(th), E_c=ec, E_c_err=err, E20=e20, per_seed=per_seed,
                                   fields=E.tolist(), angle=th.mean(axis=0).tolist())
     c = res["cells"]
     if all(k in c and c[k]["E_c"] for k in CELL_AXIS):
@@ -177,15 +195,123 @@ def analyze(root):
         if all(c[k]["E20"] for k in CELL_AXIS):
             s, t, b = (c[k]["E20"] for k in ("splay", "twist", "bend"))
             res["K1/K2 (20 deg)"], res["K3/K2 (20 deg)"] = (s / t) ** 2, (b / t) ** 2
-    any_run = next((os.path.join(root, k, d) for k in c for d in sorted(os.listdir(os.path.join(root, k)))), None)
-    if any_run:
-        p = read_input(any_run)
-        th = continuum_fields(float(p["k11"]), float(p["k22"]), float(p["k33"]), int(p["nz"]), p["_W"],
-                              float(p["eleca"]))
-        res["continuum"] = {k: dict(finite_anchoring=v[0], strong_anchoring=v[1]) for k, v in th.items()}
-        lam_ratio = {k: (th[k][0] / th["twist"][0]) ** 2 for k in ("splay", "bend")}
-        res["continuum K1/K2"], res["continuum K3/K2"] = lam_ratio["splay"], lam_ratio["bend"]
+    th = continuum_fields(k11, k22, k33, Nz, W, deps)
+    res["continuum"] = {k: dict(finite_anchoring=v[0], strong_anchoring=v[1]) for k, v in th.items()}
+    res["continuum K1/K2"] = (th["splay"][0] / th["twist"][0]) ** 2
+    res["continuum K3/K2"] = (th["bend"][0] / th["twist"][0]) ** 2
     return res
+
+
+def analyze(root):
+    """analyze_curves over a campaign directory root/<cell>/seed<s>/ of finished `mclics` runs."""
+    curves, any_run = {}, None
+    for cell, axis in CELL_AXIS.items():
+        d = os.path.join(root, cell)
+        runs = sorted(os.path.join(d, r) for r in os.listdir(d) if os.path.exists(os.path.join(d, r, "DONE"))) \
+            if os.path.isdir(d) else []
+        if runs:
+            curves[cell] = [angle_curve(r, axis) for r in runs]
+            any_run = any_run or runs[0]
+    if any_run is None:
+        return {"cells": {}}
+    p = read_input(any_run)
+    return analyze_curves(curves, float(p["k11"]), float(p["k22"]), float(p["k33"]), int(p["nz"]), p["_W"],
+                          float(p["eleca"]))
+
+
+# ---------------------------------------------------------------------------------------------------------------
+# The campaign as one batched context
+
+def lab_axis(theta_deg, phi_deg):
+    """The easy axis of an rp surface from theta_s, phi_s in degrees, as the host program computes it: FP32 radians
+    and the C library's sinf / cosf (called here through ctypes, so that the bits are the same)."""
+    import ctypes
+    import ctypes.util
+    m = ctypes.CDLL(ctypes.util.find_library("m") or "libm.so.6")
+    m.sinf.restype = m.cosf.restype = ctypes.c_float
+    m.sinf.argtypes = m.cosf.argtypes = [ctypes.c_float]
+    f = np.float32
+    to_rad = f(math.pi / 180)
+    th, ph = f(to_rad * f(theta_deg)), f(to_rad * f(phi_deg))
+    st = f(m.sinf(th))
+    return (float(st * f(m.cosf(ph))), float(st * f(m.sinf(ph))), float(f(m.cosf(th))))
+
+
+def homogeneous(theta_deg, phi_deg):
+    """The homogeneous initial director (ic.cpp:72-83): double sin / cos of the FP32 angles, rounded to FP32."""
+    t, p = float(np.float32(theta_deg)) * math.pi / 180, float(np.float32(phi_deg)) * math.pi / 180
+    return np.array([math.sin(t) * math.cos(p), math.sin(t) * math.sin(p), math.cos(t)], dtype=np.float32)
+
+
+def cell_lattice(cell, seed, T, W, deps, Nx=10, Ny=10, Nz=16):
+    """The lattice of one campaign run as `mclics` sets it up from input_lines(cell, seed, ...): the same slab
+    geometry, anchoring, GHRL coefficients, initial directors, temperature, ang_var (0.5) and seed."""
+    from . import batch, capi
+    pt, ns = batch.slab_geometry(Nx, Ny, Nz)
+    if cell in ("splay", "twist"):
+        n0 = homogeneous(90, 0)
+        surf = [("rp", W, lab_axis(90, 0)), ("rp", W, lab_axis(90, 0))]
+        field, sign = ((0.0, 0.0, 1.0) if cell == "splay" else (0.0, 1.0, 0.0)), 1.0
+    else:
+        n0 = homogeneous(0, 0)
+        surf = [("homeotropic", W, (0.0, 0.0, 0.0)), ("homeotropic", W, (0.0, 0.0, 0.0))]
+        field, sign = (0.0, 0.0, 1.0), -1.0
+    c = capi.ghrl_coefficients(K11, K22, K33, 0.0)
+    model = capi.make_model("ghrl", ghrl=(c["lam"], c["mu"], c["nu"], c["rho"], c["sigma"]), elec_dir=field,
+                            elecA=sign * deps, elecE=0.0)
+    return capi.Lattice(Nx, Ny, Nz, np.tile(n0, Nx * Ny * Nz), periodic=(1, 1, 0), pt=pt, ns=ns, surfaces=surf,
+                        model=model, T=T, ang_var=0.5, seed=seed)
+
+
+def mid_angle(ni, axis, Nx, Ny, Nz):
+    """Mean acos(|n . n0|) in degrees over the two middle layers (one for odd Nz) of a slab without empty sites."""
+    n = np.asarray(ni, dtype=np.float64).reshape(Nz, Ny, Nx, 3)[..., "xyz".index(axis)]
+    mid = [Nz // 2 - 1, Nz // 2] if Nz % 2 == 0 else [Nz // 2]
+    return float(np.degrees(np.arccos(np.minimum(1.0, np.abs(n[mid])))).mean())
+
+
+def run_batched(cells=("splay", "twist", "bend"), seeds=(1, 2), Nx=10, Ny=10, Nz=16, keep_directors=False,
+                **protocol):
+    """All runs of the campaign side by side in one context: one launch per colour pass sweeps every lattice.
+
+    Each lattice is set up as `mclics` sets up its run and keeps the same seed, so its directors follow the run's
+    trajectory bit for bit; the electric mode's schedule is the same (field set, MCT sweeps, MCS measured sweeps).
+    Returns (fields, {(cell, seed): angles}, {(cell, seed): po.dat rows}, directors per field if kept)."""
+    from . import capi
+    prm = dict(DEFAULTS)
+    prm.update({k: v for k, v in protocol.items() if v is not None})
+    runs = [(cell, s) for s in seeds for cell in cells]
+    lats = [cell_lattice(c, s, prm["T"], prm["W"], prm["deps"], Nx, Ny, Nz) for c, s in runs]
+    fields = field_values(prm["Ei"], prm["Ef"], prm["dE"])
+    angles = {r: [] for r in runs}
+    po = {r: [] for r in runs}
+    kept = []
+    t0 = last = time.time()
+    with capi.Engine(lats) as eng:
+        eng.set_graph(50)
+        for ie, E in enumerate(fields):
+            eng.set_field(E)
+            eng.sweeps(prm["MCT"])
+            eng.reset_moments()
+            eng.sweeps(prm["MCS"], 1)
+            snap = {}
+            for r, run in enumerate(runs):
+                m = eng.moments(r)
+                S1, S2, E1, E2 = m["S"] / m["n"], m["S2"] / m["n"], m["E"] / m["n"], m["E2"] / m["n"]
+                po[run].append((E, S1, S2 - S1 * S1, E1, E2 - E1 * E1))
+                d = eng.directors(r)
+                angles[run].append(mid_angle(d, CELL_AXIS[run[0]], Nx, Ny, Nz))
+                if keep_directors:
+                    snap[run] = d
+            if keep_directors:
+                kept.append(snap)
+            if time.time() - last > 30:
+                last = time.time()
+                log("  %.0f s: %d of %d field values" % (last - t0, ie + 1, len(fields)))
+    log("batched campaign: %d runs, %d field values, %.0f s" % (len(runs), len(fields), time.time() - t0))
+    return fields, angles, po, kept
+
+
 
 
 def report(res):
@@ -273,28 +399,39 @@ def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     sub = ap.add_subparsers(dest="cmd", required=True)
     r = sub.add_parser("run")
-    r.add_argument("--out", required=True)
+    r.add_argument("--out", help="campaign directory (runs of mclics)")
     r.add_argument("--cells", nargs="+", default=["splay", "twist", "bend"])
     r.add_argument("--seeds", nargs="+", type=int, default=[1, 2])
     r.add_argument("--jobs", type=int, default=6)
     for k, v in DEFAULTS.items():
         r.add_argument("--" + k, type=type(v), default=None)
     r.add_argument("--json")
+    r.add_argument("--batched", action="store_true",
+                   help="all runs in one batched context (no mclics processes, no snapshot files)")
     a = sub.add_parser("analyze")
     a.add_argument("root")
     a.add_argument("--json")
     args = ap.parse_args(argv)
-    if args.cmd == "run":
+    if args.cmd == "run" and args.batched:
+        prot = {k: getattr(args, k) for k in DEFAULTS}
+        prm = dict(DEFAULTS)
+        prm.update({k: v for k, v in prot.items() if v is not None})
+        fields, angles, po, _ = run_batched(args.cells, args.seeds, **prot)
+        curves = {c: [(fields, angles[(c, s)]) for s in args.seeds] for c in args.cells}
+        res = analyze_curves(curves, K11, K22, K33, 16, prm["W"], prm["deps"])
+        res["po"] = {"%s/seed%d" % k: v for k, v in po.items()}
+    elif args.cmd == "run":
+        if not args.out:
+            ap.error("run needs --out (or --batched)")
         prot = {k: getattr(args, k) for k in DEFAULTS}
         t0 = time.time()
         failed = run_campaign(args.out, args.cells, args.seeds, args.jobs, **prot)
         log("campaign: %.0f s" % (time.time() - t0))
         if failed:
             return 1
-        root = args.out
+        res = analyze(args.out)
     else:
-        root = args.root
-    res = analyze(root)
+        res = analyze(args.root)
     print(report(res), flush=True)
     if args.json:
         with open(args.json, "w") as f:

@@ -137,3 +137,54 @@ def test_short_campaign_through_mclics(tmp_path):
         np.testing.assert_allclose(E, [0.1, 0.65, 1.2])
         assert th[0] < 2.0, (cell, th)
         assert th[2] > 30.0, (cell, th)
+
+
+def test_field_values_follow_the_schedule():
+    """The electric mode's field values (include/sweep.h:13-34): 221 values 0.1 .. 1.2 in steps of 0.005, the last one
+    exactly Ef, each the FP32 rounding of the double schedule."""
+    E = fz.field_values(0.1, 1.2, 0.005)
+    assert len(E) == 221 and E[-1] == float(np.float32(1.2)) and E[0] == float(np.float32(0.1))
+    assert all(np.float32(e) == e for e in E)
+    assert E[100] == float(np.float32(float(np.float32(0.1)) + 100 * float(np.float32(0.005))))
+    f = lambda v: float(np.float32(v))
+    assert fz.field_values(0.1, 1.2, 0.55) == [f(0.1), f(f(0.1) + f(0.55)), f(1.2)]  # a + n d, then the end exactly
+
+
+def test_setup_matches_the_host_program():
+    """The batched runner's lattice inputs are the host program's: the rp axis of theta_s = 90, phi_s = 0 from FP32
+    sinf / cosf (x = 1, the FP32 cosine of the rounded pi/2 in z), and the homogeneous directors from double sin / cos
+    (z = cos(pi/2) rounded to FP32, not 0)."""
+    ax = fz.lab_axis(90, 0)
+    assert ax[0] == 1.0 and ax[1] == 0.0 and ax[2] == float(np.float32(math.cos(float(np.float32(math.pi / 2)))))
+    n = fz.homogeneous(90, 0)
+    assert n[0] == 1.0 and n[1] == 0.0 and n[2] == np.float32(math.cos(math.pi / 2)) and n[2] != 0
+    assert list(fz.homogeneous(0, 0)) == [0.0, 0.0, 1.0]
+
+
+@pytest.mark.gpu
+def test_batched_campaign_reproduces_the_mclics_runs(tmp_path):
+    """The three cells in one batched context against three `mclics` runs of the same input: every snapshot is the
+    same text (the directors are bit-identical), and the po.dat rows agree to the printed digits."""
+    root = str(tmp_path / "fz")
+    prot = dict(MCT=2000, MCS=50, dE=0.55)
+    print("mclics runs in", root, flush=True)
+    assert fz.run_campaign(root, seeds=(1,), jobs=3, **prot) == []
+    print("batched runs", flush=True)
+    fields, angles, po, kept = fz.run_batched(seeds=(1,), keep_directors=True, **prot)
+    assert fields == fz.field_values(0.1, 1.2, 0.55)
+    for cell in fz.CELL_AXIS:
+        d = os.path.join(root, cell, "seed1")
+        with open(os.path.join(d, "po.dat")) as f:
+            next(f)
+            rows = [[float(x) for x in l.split()] for l in f if l.strip()]
+        assert len(rows) == len(fields)
+        for ie, E in enumerate(fields):
+            idx = int(100 * (E + 1e-3 / 100))
+            with open(os.path.join(d, "director_field_%d.csv" % idx)) as f:
+                next(f)
+                cli = [l.split(",")[3:6] for l in f]
+            mine = kept[ie][(cell, 1)].reshape(-1, 3)
+            assert [["%g" % v for v in row] for row in mine] == cli, (cell, E)
+            np.testing.assert_allclose(po[(cell, 1)][ie], rows[ie], rtol=2e-5, atol=1e-9)
+        # the same angles, up to the 6 printed digits of the snapshots (acos is steep near |n . n0| = 1)
+        np.testing.assert_allclose(angles[(cell, 1)], fz.angle_curve(d, fz.CELL_AXIS[cell])[1], atol=0.01)
