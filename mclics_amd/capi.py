@@ -99,6 +99,8 @@ ABI = {
     "mcl_autotune_ex": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(TuneResult)]),
     "mcl_set_launch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]),
     "mcl_set_graph": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
+    "mcl_set_resident": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
+    "mcl_resident": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32)]),
     "mcl_mlp_create": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_uint64,
                                       ctypes.POINTER(ctypes.c_void_p)]),
     "mcl_mlp_destroy": (None, [ctypes.c_void_p]),
@@ -316,6 +318,15 @@ class Engine:
 
     def set_graph(self, block_sweeps):
         _check(self.lib.mcl_set_graph(self.h, int(block_sweeps)))
+
+    def set_resident(self, enable):
+        """mcl_set_resident: plain sweeps of small lattices in one launch per block (bit-identical trajectories)."""
+        _check(self.lib.mcl_set_resident(self.h, 1 if enable else 0))
+
+    def resident(self):
+        v = ctypes.c_int32()
+        _check(self.lib.mcl_resident(self.h, ctypes.byref(v)))
+        return bool(v.value)
 
     def synchronize(self):
         _check(self.lib.mcl_synchronize(self.h))

@@ -623,6 +623,64 @@ __device__ __forceinline__ void finalize_compact(int r, const double *__restrict
   mom[r] = m;
 }
 
+// One Metropolis trial of site s with the draws of its Philox block R (evolve.cu:162-239): rotation by pi va about x,
+// y or z, renormalization, acceptance, and the optional flip trial. With two sites per thread (SPT = 2) site e of
+// the pair takes half of R's 128 bits. Returns whether the rotation was accepted; nc and E_fin are the site's final
+// director and energy, and changed tells whether nc differs from s.n.
+template <int POT, bool SURF, bool FIELD, bool FLIP, int SPT>
+__device__ __forceinline__ bool metropolis_trial(const LatDev &L, const Site &s, const mcl::Philox4 &R, int e, float av,
+                                                 float T, float invT, float elecE,
+                                                 const SurfDev *__restrict__ surf, V3 &nc, float &E_fin,
+                                                 bool &changed) {
+  float u_ang, rot, u;
+  if constexpr (SPT == 2) {
+    const uint32_t w = R.v[2 * e];
+    u_ang = mcl::u01_open16(w >> 16);
+    rot = mcl::u01_open16(w & 0xFFFFu);
+    u = mcl::u01_open(R.v[2 * e + 1]);
+  } else {
+    u_ang = mcl::u01_open(R.v[0]);
+    rot = mcl::u01_open(R.v[1]);
+    u = mcl::u01_open(R.v[2]);
+  }
+  // trial rotation by pi * va about x, y or z (evolve.cu:162-182)
+  const float va = (2.0f * u_ang - 1.0f) * av;
+  // cos and sin of pi va: the hardware sine and cosine take their argument in turns (x 2 pi)
+  const float sn = __builtin_amdgcn_sinf(0.5f * va), cs = __builtin_amdgcn_cosf(0.5f * va);
+  const V3 n = s.n;
+  V3 nn;
+  if (rot < 0.333f) {
+    nn = V3{n.x, fmaf(n.y, cs, n.z * sn), fmaf(n.z, cs, -(n.y * sn))};
+  } else if (rot < 0.666f) {
+    nn = V3{fmaf(n.x, cs, -(n.z * sn)), n.y, fmaf(n.z, cs, n.x * sn)};
+  } else {
+    nn = V3{fmaf(n.x, cs, n.y * sn), fmaf(n.y, cs, -(n.x * sn)), n.z};
+  }
+  const float inv = __builtin_amdgcn_rsqf(dot3(nn, nn));  // |nn| ~ 1: no denormal handling needed
+  nn.x *= inv;
+  nn.y *= inv;
+  nn.z *= inv;
+  const float E_old = site_energy<POT, SURF, FIELD>(L, s, n, surf, elecE);
+  const float E_new = site_energy<POT, SURF, FIELD>(L, s, nn, surf, elecE);
+  // Metropolis; T = 0 accepts when the energy does not increase (evolve.cu:212-224)
+  const bool accepted = T > 0.0f ? (u <= __expf(-(E_new - E_old) * invT)) : (E_new <= E_old);
+  nc = accepted ? nn : n;
+  E_fin = accepted ? E_new : E_old;
+  changed = accepted;
+  if constexpr (FLIP) {
+    // separate n -> -n trial, not counted in the acceptance (evolve.cu:225-239)
+    const V3 nf{-nc.x, -nc.y, -nc.z};
+    const float E_f = site_energy<POT, SURF, FIELD>(L, s, nf, surf, elecE);
+    const float uf = mcl::u01_open(R.v[3]);
+    if (T > 0.0f ? (uf <= __expf(-(E_f - E_fin) * invT)) : (E_f <= E_fin)) {
+      nc = nf;
+      E_fin = E_f;
+      changed = true;
+    }
+  }
+  return accepted;
+}
+
 template <int POT, bool MASK, bool SURF, bool FIELD, bool FLIP, bool FREE, bool MEAS, bool ROW>
 // Launch shape (chosen by the autotuner): any multiple of 64 up to 1024 threads per workgroup; with xcd != 0
 // workgroup b runs logical tile (b & 7) * chunk + (b >> 3), so consecutive tiles share an XCD and its L2.
@@ -729,53 +787,12 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(MEAS && (M
                                                                                  SZ, s, self))
         continue;
     }
-    float u_ang, rot, u;
-    if constexpr (SPT == 2) {
-      const uint32_t w = R.v[2 * e];
-      u_ang = mcl::u01_open16(w >> 16);
-      rot = mcl::u01_open16(w & 0xFFFFu);
-      u = mcl::u01_open(R.v[2 * e + 1]);
-    } else {
-      u_ang = mcl::u01_open(R.v[0]);
-      rot = mcl::u01_open(R.v[1]);
-      u = mcl::u01_open(R.v[2]);
-    }
-    // trial rotation by pi * va about x, y or z (evolve.cu:162-182)
-    const float va = (2.0f * u_ang - 1.0f) * av;
-    // cos and sin of pi va: the hardware sine and cosine take their argument in turns (x 2 pi)
-    const float sn = __builtin_amdgcn_sinf(0.5f * va), cs = __builtin_amdgcn_cosf(0.5f * va);
-    const V3 n = s.n;
-    V3 nn;
-    if (rot < 0.333f) {
-      nn = V3{n.x, fmaf(n.y, cs, n.z * sn), fmaf(n.z, cs, -(n.y * sn))};
-    } else if (rot < 0.666f) {
-      nn = V3{fmaf(n.x, cs, -(n.z * sn)), n.y, fmaf(n.z, cs, n.x * sn)};
-    } else {
-      nn = V3{fmaf(n.x, cs, n.y * sn), fmaf(n.y, cs, -(n.x * sn)), n.z};
-    }
-    const float inv = __builtin_amdgcn_rsqf(dot3(nn, nn));  // |nn| ~ 1: no denormal handling needed
-    nn.x *= inv;
-    nn.y *= inv;
-    nn.z *= inv;
-    const float E_old = site_energy<POT, SURF, FIELD>(L, s, n, surf, elecE);
-    const float E_new = site_energy<POT, SURF, FIELD>(L, s, nn, surf, elecE);
-    // Metropolis; T = 0 accepts when the energy does not increase (evolve.cu:212-224)
-    const bool accepted = T > 0.0f ? (u <= __expf(-(E_new - E_old) * invT)) : (E_new <= E_old);
-    nacc += accepted ? 1u : 0u;
-    V3 nc = accepted ? nn : n;
-    float E_fin = accepted ? E_new : E_old;
-    bool changed = accepted;
-    if constexpr (FLIP) {
-      // separate n -> -n trial, not counted in the acceptance (evolve.cu:225-239)
-      const V3 nf{-nc.x, -nc.y, -nc.z};
-      const float E_f = site_energy<POT, SURF, FIELD>(L, s, nf, surf, elecE);
-      const float uf = mcl::u01_open(R.v[3]);
-      if (T > 0.0f ? (uf <= __expf(-(E_f - E_fin) * invT)) : (E_f <= E_fin)) {
-        nc = nf;
-        E_fin = E_f;
-        changed = true;
-      }
-    }
+    V3 nc;
+    float E_fin;
+    bool changed;
+    nacc += metropolis_trial<POT, SURF, FIELD, FLIP, SPT>(L, s, R, e, av, T, invT, elecE, surf, nc, E_fin, changed)
+                ? 1u
+                : 0u;
     if (changed) {
       stx(X, self, nc.x);
       stx(Y, self, nc.y);
@@ -846,6 +863,130 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(MEAS && (M
       for (int w = 0; w < nw; w++) v += mred[w][threadIdx.x];
       spart[(size_t)threadIdx.x * nblk + Lb] = v;
     }
+  }
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// Resident sweeps of small lattices: one workgroup per lattice holds the lattice's two colour blocks in LDS and runs
+// nsw whole sweeps in one launch, with a workgroup barrier between the colour passes instead of a kernel boundary.
+// A small lattice is otherwise bound by the latency of two launches per sweep (a 10 x 10 x 16 slab: 16 us per
+// sweep). Every trial is the one k_sweep makes: the same gather (on lattice-local positions), the same Philox
+// counters (which depend on the position only), metropolis_trial, and the same ang_var rule on the accepted count
+// of the previous sweep, so the directors follow k_sweep's trajectory bit for bit. At the end the lattice's
+// control block and counter ring hold what k_sweep's passes would have left: ang_var of the last two sweeps, their
+// accepted counts, and cleared counters for the next sweep. Plain sweeps only (measured sweeps take k_sweep).
+// One CU then does all of a lattice's work, so this pays only while no lane updates more sites than in k_sweep: at
+// most 2048 positions per colour (two sites per lane), 1024 with the flip trial (one). Measured on the MI355X: 1.3x
+// for a 10 x 10 x 16 GHRL slab and 1.4x for LL 16^3, but 0.6-0.7x for 12 x 12 x 46 and 22^3 (4-6 sites per lane).
+constexpr int kResBlock = 1024;
+constexpr int kResMax = 4096;     // colour-block positions (both colours) of a lattice: 3 x 4 bytes each in LDS
+constexpr int kResMaxFlip = 2048;  // the same with the flip trial (one site per lane)
+constexpr int kResChunk = 16384;  // sweeps per resident launch
+
+template <int POT, bool MASK, bool SURF, bool FIELD, bool FLIP, bool FREE>
+__global__ __launch_bounds__(kResBlock) void k_sweep_resident(
+    const LatDev *__restrict__ lats, CtrlDev *__restrict__ ctrl, unsigned int *__restrict__ cnt,
+    const unsigned long long *__restrict__ sweep_base, unsigned k0, int nsw, float *__restrict__ X,
+    float *__restrict__ Y, float *__restrict__ Z, const uint8_t *__restrict__ PT, const float *__restrict__ SX,
+    const float *__restrict__ SY, const float *__restrict__ SZ, const SurfDev *__restrict__ surf) {
+  __shared__ float sd[3 * kResMax];
+  __shared__ unsigned acc[2];
+  const int r = blockIdx.x;
+  LatDev L = lats[r];
+  const uint32_t off = L.off;
+  L.off = 0;  // gather on lattice-local positions: colour 0 at [0, nc), colour 1 at [nc, 2 nc)
+  const int n2 = 2 * L.nc;
+  float *const LX = sd, *const LY = sd + kResMax, *const LZ = sd + 2 * kResMax;
+  for (int i = (int)threadIdx.x; i < n2; i += kResBlock) {
+    LX[i] = X[off + i];
+    LY[i] = Y[off + i];
+    LZ[i] = Z[off + i];
+  }
+  const uint8_t *const PTl = MASK ? PT + off : nullptr;
+  const float *const SXl = SURF ? SX + off : nullptr;
+  const float *const SYl = SURF ? SY + off : nullptr;
+  const float *const SZl = SURF ? SZ + off : nullptr;
+  const float T = ctrl[r].T, invT = ctrl[r].invT, elecE = ctrl[r].elecE;
+  const float av_set = ctrl[r].av_set;
+  const unsigned long long av_set_sweep = ctrl[r].av_set_sweep;
+  const unsigned long long sw0 = *sweep_base + k0;
+  unsigned int *const cnt_r = cnt + (size_t)r * kCntLattice;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  // the previous sweep: its ang_var and, when the rule needs it, its accepted count (summed over the 64 counters)
+  float av_prev = ctrl[r].av_store[(sw0 - 1) & 1];
+  unsigned c_prev = 0;
+  if (sw0 != av_set_sweep) {
+    unsigned a = cnt_r[((sw0 - 1) % 3) * kCntSweep + lane * kCntStride];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o, 64);
+    c_prev = (unsigned)__builtin_amdgcn_readfirstlane(a);
+  }
+  if (threadIdx.x < 2) acc[threadIdx.x] = 0u;
+  __syncthreads();
+  float av_m1 = av_prev;  // ang_var and accepted count of the sweep before the last one run here
+  unsigned c_m1 = c_prev;
+  for (int k = 0; k < nsw; k++) {
+    const unsigned long long sweep = sw0 + (unsigned long long)k;
+    const float av = sweep == av_set_sweep ? av_set : ang_var_rule(av_prev, c_prev, L.nvalid);
+    for (int colour = 0; colour < 2; colour++) {
+      unsigned nacc = 0;
+      constexpr int SPT = FLIP ? 1 : 2;
+      // positions [base, base + 64 SPT) per wave and step, as the waves of k_sweep take them
+      for (int base = 64 * SPT * wave; base < L.nc; base += 64 * SPT * (kResBlock / 64)) {
+        const uint32_t ctr0 = SPT == 2 ? (uint32_t)((base >> 7) * 64 + lane) : (uint32_t)(base + lane);
+        const mcl::Philox4 R = mcl::philox4x32_10(ctr0, (uint32_t)sweep, (uint32_t)(sweep >> 32),
+                                                  kTagSweep + (uint32_t)colour, L.key0, L.key1);
+#pragma unroll
+        for (int e = 0; e < SPT; e++) {
+          const int t = base + lane + 64 * e;
+          Site s;
+          uint32_t self = 0;
+          if (t >= L.nc ||
+              !gather<MASK, SURF, FREE, false, POT != MCL_POT_LL>(L, colour, t, LX, LY, LZ, PTl, SXl, SYl, SZl, s,
+                                                                   self))
+            continue;
+          V3 nc;
+          float E_fin;
+          bool changed;
+          nacc += metropolis_trial<POT, SURF, FIELD, FLIP, SPT>(L, s, R, e, av, T, invT, elecE, surf, nc, E_fin,
+                                                                changed)
+                      ? 1u
+                      : 0u;
+          if (changed) {
+            LX[self] = nc.x;
+            LY[self] = nc.y;
+            LZ[self] = nc.z;
+          }
+        }
+      }
+      unsigned wc = 0;
+      for (int bit = 0; bit < 16; bit++) wc += (unsigned)__popcll(__ballot((nacc >> bit) & 1u)) << bit;
+      if (lane == 0 && wc) atomicAdd(&acc[k & 1], wc);
+      __syncthreads();
+      // acc[(k + 1) & 1] was last read at the end of sweep k - 1, before the barrier above; the next barrier
+      // orders this reset before the atomics of sweep k + 1
+      if (colour == 0 && threadIdx.x == 0) acc[(k + 1) & 1] = 0u;
+    }
+    av_m1 = av_prev;
+    c_m1 = c_prev;
+    av_prev = av;
+    c_prev = acc[k & 1];
+  }
+  for (int i = (int)threadIdx.x; i < n2; i += kResBlock) {
+    X[off + i] = LX[i];
+    Y[off + i] = LY[i];
+    Z[off + i] = LZ[i];
+  }
+  if (nsw > 0 && threadIdx.x < kCnt) {
+    const unsigned long long last = sw0 + (unsigned long long)(nsw - 1);
+    const uint32_t q = threadIdx.x * kCntStride;
+    if (threadIdx.x == 0) {
+      ctrl[r].av_store[last & 1] = av_prev;
+      if (nsw >= 2) ctrl[r].av_store[(last - 1) & 1] = av_m1;
+    }
+    cnt_r[(last % 3) * kCntSweep + q] = threadIdx.x == 0 ? c_prev : 0u;
+    if (nsw >= 2) cnt_r[((last - 1) % 3) * kCntSweep + q] = threadIdx.x == 0 ? c_m1 : 0u;
+    cnt_r[((last + 1) % 3) * kCntSweep + q] = 0u;
   }
 }
 
@@ -1251,6 +1392,8 @@ struct Ctx {
   bool free = false;  // some lattice has a free (non-periodic) axis
   bool row = false;   // every lattice is row-aligned: nx a multiple of 128 (gather_row)
   bool zero_empty = false;  // LL context with masks: empty sites are stored as zero vectors (see gather's NBPT)
+  bool resident_ok = false;  // every lattice fits the LDS of one workgroup (k_sweep_resident)
+  bool resident_on = true;   // plain sweeps of such a context run resident (mcl_set_resident)
   bool xq = false;  // radial or pear observables are measured
   std::vector<LatHost> lh;
   std::vector<LatDev> ld;
@@ -1317,6 +1460,35 @@ void launch_sweep(dim3 g, dim3 b, hipStream_t st, const Ctx &c, int colour, unsi
 // Grid of a sweep launch: the work list, plus one workgroup per lattice for the deferred finalize of a measurement
 // folded into the previous sweep.
 dim3 sweep_grid(const Ctx *c) { return dim3(c->xcd ? 8 * c->chunk : c->nblk + (c->fused ? c->nlat : 0)); }
+using ResFn = void (*)(hipStream_t, const Ctx &, unsigned, int);
+template <int POT, bool MASK, bool SURF, bool FIELD, bool FLIP, bool FREE>
+void launch_resident(hipStream_t st, const Ctx &c, unsigned k0, int nsw) {
+  hipLaunchKernelGGL((k_sweep_resident<POT, MASK, SURF, FIELD, FLIP, FREE>), dim3(c.nlat), dim3(kResBlock), 0, st,
+                     c.lat_d, c.ctrl_d, c.cnt, c.sweep_base, k0, nsw, c.X, c.Y, c.Z, c.PT, c.SX, c.SY, c.SZ,
+                     c.surf_d);
+}
+template <int POT, bool MASK, bool SURF, bool FREE>
+ResFn pick_resident_field(bool field, bool flip) {
+  if (field)
+    return flip ? launch_resident<POT, MASK, SURF, true, true, FREE> : launch_resident<POT, MASK, SURF, true, false, FREE>;
+  return flip ? launch_resident<POT, MASK, SURF, false, true, FREE> : launch_resident<POT, MASK, SURF, false, false, FREE>;
+}
+// the same instance choice as pick_sweep
+template <int POT>
+ResFn pick_resident_geom(bool mask, bool surf, bool free, bool field, bool flip) {
+  if (surf) return pick_resident_field<POT, true, true, true>(field, flip);
+  if (mask) return pick_resident_field<POT, true, false, true>(field, flip);
+  if (free) return pick_resident_field<POT, false, false, true>(field, flip);
+  return pick_resident_field<POT, false, false, false>(field, flip);
+}
+ResFn pick_resident(const Ctx &c) {
+  switch (c.pot) {
+    case MCL_POT_GHRL: return pick_resident_geom<MCL_POT_GHRL>(c.mask, c.surf, c.free, c.field, c.flip);
+    case MCL_POT_PEAR: return pick_resident_geom<MCL_POT_PEAR>(c.mask, c.surf, c.free, c.field, c.flip);
+    default: return pick_resident_geom<MCL_POT_LL>(c.mask, c.surf, c.free, c.field, c.flip);
+  }
+}
+
 template <int POT, bool MASK, bool SURF, bool FIELD, bool FREE>
 void launch_measure(dim3 g, dim3 b, hipStream_t st, const Ctx &c) {
   if (c.xq)
@@ -1635,7 +1807,9 @@ int create_ctx(const mcl_lattice_desc *lat, int nlat, Ctx *c) {
   c->fused = !c->xq;
   c->row = true;
   c->zero_empty = c->pot == MCL_POT_LL && c->mask;
+  c->resident_ok = true;
   for (int r = 0; r < nlat; r++) {
+    if (2ll * c->ld[r].nc > (c->flip ? kResMaxFlip : kResMax)) c->resident_ok = false;
     if (c->ld[r].nx < 2) c->fused = false;
     if ((long long)c->ld[r].nvalid != (long long)c->ld[r].nx * c->ld[r].ny * c->ld[r].nz) c->fused = false;
     if (c->ld[r].nx % 128 != 0) c->row = false;
@@ -1876,6 +2050,15 @@ int mcl_sweeps(void *ctx, int32_t nsweeps, int32_t measure_every) {
   if (!c) return fail(MCL_ERR_ARG, "null context");
   if (nsweeps < 0 || measure_every < 0) return fail(MCL_ERR_ARG, "negative sweep count");
   if (nsweeps == 0) return MCL_OK;
+  if (measure_every == 0 && c->resident_ok && c->resident_on && c->timing_stride == 0) {
+    // plain sweeps of small lattices: whole sweeps in one launch per kResChunk, bit-identical to the passes below
+    const ResFn rf = pick_resident(*c);
+    for (int k0 = 0; k0 < nsweeps; k0 += kResChunk) rf(c->stream, *c, (unsigned)k0, std::min(kResChunk, nsweeps - k0));
+    hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, c->stream, c->sweep_base, (unsigned)nsweeps);
+    c->sweeps_done += (unsigned long long)nsweeps;
+    MCL_HIP(hipGetLastError());
+    return MCL_OK;
+  }
   const SweepFn sf = pick_sweep(*c);
   const dim3 grid = sweep_grid(c), block(c->bs);
   int done = 0;
@@ -2278,6 +2461,20 @@ int mcl_set_graph(void *ctx, int32_t block_sweeps) {
   if (!c) return fail(MCL_ERR_ARG, "null context");
   if (block_sweeps < 0) return fail(MCL_ERR_ARG, "negative graph block");
   c->graph_block = block_sweeps;
+  return MCL_OK;
+}
+
+int mcl_set_resident(void *ctx, int32_t enable) {
+  Ctx *c = as_ctx(ctx);
+  if (!c) return fail(MCL_ERR_ARG, "null context");
+  c->resident_on = enable != 0;
+  return MCL_OK;
+}
+
+int mcl_resident(void *ctx, int32_t *active) {
+  Ctx *c = as_ctx(ctx);
+  if (!c || !active) return fail(MCL_ERR_ARG, "null argument");
+  *active = c->resident_ok && c->resident_on ? 1 : 0;
   return MCL_OK;
 }
 
