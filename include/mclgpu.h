@@ -205,8 +205,8 @@ int mcl_set_graph(void *ctx, int32_t block_sweeps);
  * whose sweeps are otherwise bound by the latency of two launches each). When every lattice of the context has at
  * most 4096 colour-block positions, 2 ceil(nx/2) ny nz (2048 with the flip trial: 16^3, 10 x 10 x 16 and 6 x 6 x 48
  * fit), mcl_sweeps with measure_every = 0 runs whole sweeps with each lattice held in the LDS of one workgroup, many
- * sweeps per launch. The trajectories are bit-identical to the two-pass path. On by default; enable = 0 switches it
- * off. */
+ * sweeps per launch (one position per thread up to 1024 per colour). The trajectories are bit-identical to the
+ * two-pass path. On by default; enable = 0 switches it off. */
 int mcl_set_resident(void *ctx, int32_t enable);
 /* *active = 1 when plain sweeps of the context run resident (the context qualifies and it is switched on). */
 int mcl_resident(void *ctx, int32_t *active);

@@ -883,7 +883,10 @@ constexpr int kResMax = 4096;     // colour-block positions (both colours) of a 
 constexpr int kResMaxFlip = 2048;  // the same with the flip trial (one site per lane)
 constexpr int kResChunk = 16384;  // sweeps per resident launch
 
-template <int POT, bool MASK, bool SURF, bool FIELD, bool FLIP, bool FREE>
+// ONE (no flip trial, every lattice at most kResBlock positions per colour): each thread takes one position and
+// computes the draw of its pair (p, p + 64) itself, the half k_sweep gives that position, so no lane runs two
+// trials in a row and twice the waves hide the latency (a separate instance: both ways in one kernel spill).
+template <int POT, bool MASK, bool SURF, bool FIELD, bool FLIP, bool FREE, bool ONE>
 __global__ __launch_bounds__(kResBlock) void k_sweep_resident(
     const LatDev *__restrict__ lats, CtrlDev *__restrict__ ctrl, unsigned int *__restrict__ cnt,
     const unsigned long long *__restrict__ sweep_base, unsigned k0, int nsw, float *__restrict__ X,
@@ -931,31 +934,55 @@ __global__ __launch_bounds__(kResBlock) void k_sweep_resident(
     for (int colour = 0; colour < 2; colour++) {
       unsigned nacc = 0;
       constexpr int SPT = FLIP ? 1 : 2;
-      // positions [base, base + 64 SPT) per wave and step, as the waves of k_sweep take them
-      for (int base = 64 * SPT * wave; base < L.nc; base += 64 * SPT * (kResBlock / 64)) {
-        const uint32_t ctr0 = SPT == 2 ? (uint32_t)((base >> 7) * 64 + lane) : (uint32_t)(base + lane);
-        const mcl::Philox4 R = mcl::philox4x32_10(ctr0, (uint32_t)sweep, (uint32_t)(sweep >> 32),
-                                                  kTagSweep + (uint32_t)colour, L.key0, L.key1);
-#pragma unroll
-        for (int e = 0; e < SPT; e++) {
-          const int t = base + lane + 64 * e;
-          Site s;
-          uint32_t self = 0;
-          if (t >= L.nc ||
-              !gather<MASK, SURF, FREE, false, POT != MCL_POT_LL>(L, colour, t, LX, LY, LZ, PTl, SXl, SYl, SZl, s,
-                                                                   self))
-            continue;
+      if constexpr (ONE) {
+        const int t = (int)threadIdx.x;
+        Site s;
+        uint32_t self = 0;
+        if (t < L.nc &&
+            gather<MASK, SURF, FREE, false, POT != MCL_POT_LL>(L, colour, t, LX, LY, LZ, PTl, SXl, SYl, SZl, s, self)) {
+          const uint32_t ctr0 = (uint32_t)((t >> 7) * 64 + (t & 63));
+          const mcl::Philox4 R = mcl::philox4x32_10(ctr0, (uint32_t)sweep, (uint32_t)(sweep >> 32),
+                                                    kTagSweep + (uint32_t)colour, L.key0, L.key1);
           V3 nc;
           float E_fin;
           bool changed;
-          nacc += metropolis_trial<POT, SURF, FIELD, FLIP, SPT>(L, s, R, e, av, T, invT, elecE, surf, nc, E_fin,
-                                                                changed)
-                      ? 1u
-                      : 0u;
+          nacc = metropolis_trial<POT, SURF, FIELD, FLIP, SPT>(L, s, R, (t >> 6) & 1, av, T, invT, elecE, surf, nc,
+                                                               E_fin, changed)
+                     ? 1u
+                     : 0u;
           if (changed) {
             LX[self] = nc.x;
             LY[self] = nc.y;
             LZ[self] = nc.z;
+          }
+        }
+      } else {
+        // positions [base, base + 64 SPT) per wave and step, as the waves of k_sweep take them
+        for (int base = 64 * SPT * wave; base < L.nc; base += 64 * SPT * (kResBlock / 64)) {
+          const uint32_t ctr0 = SPT == 2 ? (uint32_t)((base >> 7) * 64 + lane) : (uint32_t)(base + lane);
+          const mcl::Philox4 R = mcl::philox4x32_10(ctr0, (uint32_t)sweep, (uint32_t)(sweep >> 32),
+                                                    kTagSweep + (uint32_t)colour, L.key0, L.key1);
+  #pragma unroll
+          for (int e = 0; e < SPT; e++) {
+            const int t = base + lane + 64 * e;
+            Site s;
+            uint32_t self = 0;
+            if (t >= L.nc ||
+                !gather<MASK, SURF, FREE, false, POT != MCL_POT_LL>(L, colour, t, LX, LY, LZ, PTl, SXl, SYl, SZl, s,
+                                                                     self))
+              continue;
+            V3 nc;
+            float E_fin;
+            bool changed;
+            nacc += metropolis_trial<POT, SURF, FIELD, FLIP, SPT>(L, s, R, e, av, T, invT, elecE, surf, nc, E_fin,
+                                                                  changed)
+                        ? 1u
+                        : 0u;
+            if (changed) {
+              LX[self] = nc.x;
+              LY[self] = nc.y;
+              LZ[self] = nc.z;
+            }
           }
         }
       }
@@ -1394,6 +1421,7 @@ struct Ctx {
   bool zero_empty = false;  // LL context with masks: empty sites are stored as zero vectors (see gather's NBPT)
   bool resident_ok = false;  // every lattice fits the LDS of one workgroup (k_sweep_resident)
   bool resident_on = true;   // plain sweeps of such a context run resident (mcl_set_resident)
+  bool resident_one = false;  // ... with at most kResBlock positions per colour and no flip trial (ONE)
   bool xq = false;  // radial or pear observables are measured
   std::vector<LatHost> lh;
   std::vector<LatDev> ld;
@@ -1463,8 +1491,16 @@ dim3 sweep_grid(const Ctx *c) { return dim3(c->xcd ? 8 * c->chunk : c->nblk + (c
 using ResFn = void (*)(hipStream_t, const Ctx &, unsigned, int);
 template <int POT, bool MASK, bool SURF, bool FIELD, bool FLIP, bool FREE>
 void launch_resident(hipStream_t st, const Ctx &c, unsigned k0, int nsw) {
-  hipLaunchKernelGGL((k_sweep_resident<POT, MASK, SURF, FIELD, FLIP, FREE>), dim3(c.nlat), dim3(kResBlock), 0, st,
-                     c.lat_d, c.ctrl_d, c.cnt, c.sweep_base, k0, nsw, c.X, c.Y, c.Z, c.PT, c.SX, c.SY, c.SZ,
+  if constexpr (!FLIP) {
+    if (c.resident_one) {
+      hipLaunchKernelGGL((k_sweep_resident<POT, MASK, SURF, FIELD, FLIP, FREE, true>), dim3(c.nlat), dim3(kResBlock),
+                         0, st, c.lat_d, c.ctrl_d, c.cnt, c.sweep_base, k0, nsw, c.X, c.Y, c.Z, c.PT, c.SX, c.SY,
+                         c.SZ, c.surf_d);
+      return;
+    }
+  }
+  hipLaunchKernelGGL((k_sweep_resident<POT, MASK, SURF, FIELD, FLIP, FREE, false>), dim3(c.nlat), dim3(kResBlock), 0,
+                     st, c.lat_d, c.ctrl_d, c.cnt, c.sweep_base, k0, nsw, c.X, c.Y, c.Z, c.PT, c.SX, c.SY, c.SZ,
                      c.surf_d);
 }
 template <int POT, bool MASK, bool SURF, bool FREE>
@@ -1808,8 +1844,10 @@ int create_ctx(const mcl_lattice_desc *lat, int nlat, Ctx *c) {
   c->row = true;
   c->zero_empty = c->pot == MCL_POT_LL && c->mask;
   c->resident_ok = true;
+  c->resident_one = !c->flip;
   for (int r = 0; r < nlat; r++) {
     if (2ll * c->ld[r].nc > (c->flip ? kResMaxFlip : kResMax)) c->resident_ok = false;
+    if (c->ld[r].nc > kResBlock) c->resident_one = false;
     if (c->ld[r].nx < 2) c->fused = false;
     if ((long long)c->ld[r].nvalid != (long long)c->ld[r].nx * c->ld[r].ny * c->ld[r].nz) c->fused = false;
     if (c->ld[r].nx % 128 != 0) c->row = false;

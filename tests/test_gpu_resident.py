@@ -36,6 +36,13 @@ def _cases(capi):
                                                T=0.7, seed=10)]
     out["ll_batch"] = [capi.Lattice(n, n, n, rnd(ones(n ** 3), 20 + n), model=capi.make_model("ll"),
                                     T=0.9 + 0.05 * n / 4, seed=30 + n) for n in (4, 8, 10, 12, 14, 16)]
+    # at most 1024 positions per colour: one position per thread (its own instance)
+    out["ll_batch_small"] = [capi.Lattice(n, n, n, rnd(ones(n ** 3), 40 + n), model=capi.make_model("ll"),
+                                          T=1.0, seed=50 + n) for n in (4, 6, 8, 10, 12)]
+    pt, ns = batch.sphere_geometry(12)
+    out["ghrl_chiral_sphere_12"] = [capi.Lattice(12, 12, 12, rnd(pt, 11), periodic=(0, 0, 0), pt=pt, ns=ns,
+                                                 surfaces=[("homeotropic", 1.5, (0, 0, 0))],
+                                                 model=_ghrl(capi, (1.0, 0.6, 1.4, 12.0)), T=0.5, seed=12)]
     return out
 
 
@@ -60,7 +67,8 @@ def _run(capi, lats, resident):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("name", ["ll_bulk_16", "fz_splay_slab", "fz_bend_slab_T", "pear_flip_12", "ll_sphere_14",
-                                  "ghrl_chiral_sphere_16", "ll_odd_free_9x10x12", "ll_batch"])
+                                  "ghrl_chiral_sphere_16", "ll_odd_free_9x10x12", "ll_batch", "ll_batch_small",
+                                  "ghrl_chiral_sphere_12"])
 def test_resident_sweeps_are_bit_identical(mcl, name):
     print("resident case", name, flush=True)
     lats = _cases(mcl)[name]
