@@ -116,6 +116,9 @@ GpuRun::GpuRun(Params &p, Lattice &L) : p_(p), L_(L) {
   // directors stay bit-identical, but the FP64 sums of the measurements follow the workgroups, so their last bits
   // can differ between runs that pick different shapes.
   if (env_flag("MCLICS_GRAPH", true)) check(mcl_set_graph(ctx_, 50), "setup");
+  // Small lattices run their plain sweeps resident in one workgroup's LDS (bit-identical); MCLICS_RESIDENT=0 keeps
+  // them on the two-pass kernels.
+  if (!env_flag("MCLICS_RESIDENT", true)) check(mcl_set_resident(ctx_, 0), "setup");
   if (env_flag("MCLICS_AUTOTUNE", false)) {
     mcl_tune_result r;
     check(mcl_autotune(ctx_, 20, &r), "autotune");

@@ -42,7 +42,7 @@ def run(tmp_path):
     assert os.access(PROGRAM, os.X_OK), "mclics is not built (python -m mclics_amd.build)"
     count = [0]
 
-    def go(params, files=None, setup=None, raw=False):
+    def go(params, files=None, setup=None, raw=False, env=None):
         count[0] += 1
         d = tmp_path / ("run%03d" % count[0])
         d.mkdir()
@@ -58,7 +58,8 @@ def run(tmp_path):
             setup(str(d))
         print("mclics run in %s: %s" % (d, dict(params) if isinstance(params, dict) else params), flush=True)
         p = subprocess.run([PROGRAM, "input.txt"], cwd=str(d), stdout=subprocess.PIPE, stderr=subprocess.PIPE,
-                           universal_newlines=True, errors="replace", timeout=300)
+                           universal_newlines=True, errors="replace", timeout=300,
+                           env=dict(os.environ, **(env or {})))
         return Run(str(d), p)
 
     return go
@@ -357,3 +358,22 @@ def test_custom_geometry_ground_state(run):
     close("S", rows[0]["S"], 1.0, 1e-5)
     pts = {int(row[2]): int(row[7]) for row in read_ic(r)}
     assert pts == {0: 0, 1: 1, 2: 1, 3: 2, 4: 1, 5: 1}
+
+
+@pytestmark_gpu
+def test_launch_switches_leave_outputs_identical(run):
+    """The program's launch switches change how sweeps are launched, never their results: graph replay off
+    (MCLICS_GRAPH=0) and resident sweeps off (MCLICS_RESIDENT=0) give the same po.dat and snapshots, byte for
+    byte, on a lattice small enough for the resident sweeps (12^3: 864 positions per colour) and a GHRL slab."""
+    cases = [base(Nx=12, Ny=12, Nz=12, ic="random", Ti=1.2, Tf=1.0, dT=-0.1, MCT=300, MCS=40),
+             plus(base(Nx=10, Ny=10, Nz=16, geometry="slab", zbound="free", ic="random", Ti=0.6, Tf=0.5, dT=-0.1,
+                       MCT=200, MCS=30, potential="ghrl"), "k11 1.48", "k22 1", "k33 2.28",
+                  "anchoring_type 0 homeotropic", "W 0 2", "anchoring_type 1 homeotropic", "W 1 2")]
+    for params in cases:
+        outs = []
+        for env in ({}, {"MCLICS_RESIDENT": "0"}, {"MCLICS_GRAPH": "0", "MCLICS_RESIDENT": "0"}):
+            r = run(params, env=env).ok()
+            files = ["po.dat"] + r.snapshots()
+            outs.append({f: open(os.path.join(r.dir, f)).read() for f in files})
+        assert len(outs[0]) >= 3
+        assert outs[0] == outs[1] == outs[2]
