@@ -30,6 +30,11 @@ Two further workloads print a line of the same form (the default run prints only
                      the best of the 0-sweep runs (same setup, ic.csv and snapshot), so setup and output are left
                      out; N is larger than the reference's so that the sweeps dominate on this GPU. One JSON line
                      with a row per case, beside the reference's RTX 4070 figures from its results.csv.
+  --workload smalllat  the small lattices of the reference's paper campaigns, whose sweeps are bound by launch
+                     latency: the six 10x10x16 GHRL slabs of the Freedericksz campaign (E7 constants, W = 10,
+                     T = 0.0008, field 0.6) in one context, and one LL 16^3 bulk at T = 1.1, --steps plain sweeps
+                     each (graph replay as --graph sets), timed with the resident sweeps (each lattice held in one
+                     workgroup's LDS) and with the two-pass kernels; the directors are compared bit for bit.
 """
 import argparse
 import json
@@ -315,6 +320,51 @@ def run_refcases(a, repeat=2):
             "cases": rows}
 
 
+def run_smalllat(a):
+    """Plain sweeps of small lattices, resident against two-pass (the same trajectories)."""
+    from mclics_amd import batch, capi, freedericksz
+    rng_dirs = lambda n, seed: batch.random_directors(np.ones(n, dtype=np.int32), seed)
+    cases = [("six 10x10x16 GHRL slabs (Freedericksz cells, seeds 1 and 2)",
+              lambda: [freedericksz.cell_lattice(c, s, 0.0008, 10.0, 0.25) for s in (1, 2)
+                       for c in ("splay", "twist", "bend")], 0.6),
+             ("LL 16^3 bulk, T = 1.1", lambda: [capi.Lattice(16, 16, 16, rng_dirs(4096, 1), model=capi.make_model("ll"),
+                                                             T=1.1, seed=1)], None)]
+    rows = []
+    for name, make, field in cases:
+        row = {"case": name}
+        dirs = {}
+        for resident in (True, False):
+            lats = make()
+            sites = sum(l.dims[0] * l.dims[1] * l.dims[2] for l in lats)
+            with capi.Engine(lats) as eng:
+                eng.set_graph(a.graph)
+                eng.set_resident(resident)
+                if field is not None:
+                    eng.set_field(field)
+                eng.sweeps(a.warmup)
+                eng.synchronize()
+                t0 = time.perf_counter()
+                eng.sweeps(a.steps)
+                eng.synchronize()
+                dt = time.perf_counter() - t0
+                dirs[resident] = [eng.directors(r) for r in range(len(lats))]
+            key = "resident" if resident else "two_pass"
+            row[key + "_us_per_sweep"] = round(dt / a.steps * 1e6, 3)
+            row[key + "_site_updates_per_ns"] = round(a.steps * sites / (dt * 1e9), 4)
+        row["speedup"] = round(row["two_pass_us_per_sweep"] / row["resident_us_per_sweep"], 3)
+        row["bit_identical"] = all(np.array_equal(x, y) for x, y in zip(dirs[True], dirs[False]))
+        log("bench: %s: %.2f us per sweep resident, %.2f two-pass (x%.2f), identical %s" % (
+            name, row["resident_us_per_sweep"], row["two_pass_us_per_sweep"], row["speedup"], row["bit_identical"]))
+        rows.append(row)
+    first = rows[0]
+    return {"metric": "Metropolis site-updates/ns of small lattices (resident sweeps)",
+            "value": first["resident_site_updates_per_ns"], "unit": "site-updates/ns", "n_gpus": 1, "steps": a.steps,
+            "warmup": a.warmup, "ms_per_step": round(first["resident_us_per_sweep"] / 1e3, 6),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": "small lattices of the paper campaigns, plain sweeps", "cases": rows},
+            "roofline": None, "cpu_baseline": None}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -324,7 +374,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-bytes", type=int, default=256 << 20,
                     help="size of the STREAM-style copy that checks the HBM peak (0: skip)")
-    ap.add_argument("--workload", choices=("ll128", "slab", "batch", "refcases"), default="ll128")
+    ap.add_argument("--workload", choices=("ll128", "slab", "batch", "refcases", "smalllat"), default="ll128")
     ap.add_argument("--launch", default=None, metavar="THREADS,XCD_ORDER",
                     help="fixed launch shape instead of the autotuner's choice (for profiling runs, whose tracer "
                          "changes the timings the autotuner compares)")
@@ -341,7 +391,8 @@ def main():
     from mclics_amd import capi
     capi.load()
     if a.workload != "ll128":
-        out = {"slab": run_slab, "batch": run_batch_workload, "refcases": run_refcases}[a.workload](a)
+        out = {"slab": run_slab, "batch": run_batch_workload, "refcases": run_refcases,
+               "smalllat": run_smalllat}[a.workload](a)
         print(json.dumps(out), flush=True)
         return
     nt = N ** 3
