@@ -6,7 +6,7 @@ then used to
   - place temperature points around its predicted peak of the susceptibility chi = N varS / T, and
   - warm-start the directors of each point from the predicted order parameter S (a Maier-Saupe distribution
     around z with <P2> = S).
-Every point is afterwards run with the plain method, which produces every reported number and judges the
+Every point is afterwards run with the plain method, which produces every reported number and measures the
 surrogate's error (tni.py). The Metropolis rule itself is never changed.
 
     python -m mclics_amd.surrogate generate --runs 25 --out data.npz     # batched plain runs -> pairs
