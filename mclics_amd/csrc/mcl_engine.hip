@@ -962,7 +962,7 @@ __global__ __launch_bounds__(kResBlock) void k_sweep_resident(
           const uint32_t ctr0 = SPT == 2 ? (uint32_t)((base >> 7) * 64 + lane) : (uint32_t)(base + lane);
           const mcl::Philox4 R = mcl::philox4x32_10(ctr0, (uint32_t)sweep, (uint32_t)(sweep >> 32),
                                                     kTagSweep + (uint32_t)colour, L.key0, L.key1);
-  #pragma unroll
+#pragma unroll
           for (int e = 0; e < SPT; e++) {
             const int t = base + lane + 64 * e;
             Site s;
