@@ -681,7 +681,9 @@ __device__ __forceinline__ bool metropolis_trial(const LatDev &L, const Site &s,
   return accepted;
 }
 
-template <int POT, bool MASK, bool SURF, bool FIELD, bool FLIP, bool FREE, bool MEAS, bool ROW>
+// EMPTY = false (masked GHRL and pear contexts none of whose lattices has an empty site, such as the slab): the
+// neighbours' point types are not loaded, since none of them can be 0 (LL never loads them, see gather's NBPT).
+template <int POT, bool MASK, bool SURF, bool FIELD, bool FLIP, bool FREE, bool MEAS, bool ROW, bool EMPTY = true>
 // Launch shape (chosen by the autotuner): any multiple of 64 up to 1024 threads per workgroup; with xcd != 0
 // workgroup b runs logical tile (b & 7) * chunk + (b >> 3), so consecutive tiles share an XCD and its L2.
 // The sweep number is *sweep_base + k: graphs replay fixed arguments while the base advances on the device.
@@ -778,13 +780,13 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(MEAS && (M
       const uint32_t g = (uint32_t)bd.y + (uint32_t)(64 * SPT * __builtin_amdgcn_readfirstlane(threadIdx.x >> 6)) +
                          (uint32_t)(64 * e);
       if (g >= (uint32_t)L.nc ||
-          !gather_row<MASK, SURF, FREE, MEAS && MASK, POT != MCL_POT_LL>(L, colour, g, lane, X, Y, Z, PT, SX, SY,
-                                                                          SZ, s, self))
+          !gather_row<MASK, SURF, FREE, MEAS && MASK, EMPTY && POT != MCL_POT_LL>(L, colour, g, lane, X, Y, Z, PT,
+                                                                                   SX, SY, SZ, s, self))
         continue;
     } else {
       const int t = tw + 64 * e;
-      if (t >= L.nc || !gather<MASK, SURF, FREE, MEAS && MASK, POT != MCL_POT_LL>(L, colour, t, X, Y, Z, PT, SX, SY,
-                                                                                 SZ, s, self))
+      if (t >= L.nc || !gather<MASK, SURF, FREE, MEAS && MASK, EMPTY && POT != MCL_POT_LL>(L, colour, t, X, Y, Z, PT,
+                                                                                          SX, SY, SZ, s, self))
         continue;
     }
     V3 nc;
@@ -1445,6 +1447,7 @@ struct Ctx {
   double *partials = nullptr;
   // measurement folded into the colour-1 sweep pass (unmasked lattices without the radial or pear observables)
   bool fused = false;
+  bool has_empty = false;  // some lattice has an empty site (masked GHRL / pear sweeps then check neighbours)
   double *spart = nullptr;  // kNQS partials per workgroup of the sweep work list
   size_t spart_cap = 0;     // workgroups spart holds
   int2 *sr_d = nullptr;     // per lattice: its workgroups [first, last) in the sweep work list
@@ -1464,6 +1467,15 @@ using MeasFn = void (*)(dim3, dim3, hipStream_t, const Ctx &);
 
 template <int POT, bool MASK, bool SURF, bool FIELD, bool FLIP, bool FREE, bool MEAS, bool ROW>
 void launch_sweep_k(dim3 g, dim3 b, hipStream_t st, const Ctx &c, int colour, unsigned k, int fin) {
+  if constexpr (MASK && POT != MCL_POT_LL) {
+    if (!c.has_empty) {
+      hipLaunchKernelGGL((k_sweep<POT, MASK, SURF, FIELD, FLIP, FREE, MEAS, ROW, false>), g, b, 0, st, c.lat_d,
+                         c.ctrl_d, c.cnt, c.blk_d, c.nblk, c.chunk, c.xcd, colour, c.sweep_base, k, c.X, c.Y, c.Z,
+                         c.PT, c.SX, c.SY, c.SZ, c.surf_d, c.spart, fin, c.fused ? c.nlat : 0, c.sr_d, c.mom_d,
+                         c.obs_d);
+      return;
+    }
+  }
   hipLaunchKernelGGL((k_sweep<POT, MASK, SURF, FIELD, FLIP, FREE, MEAS, ROW>), g, b, 0, st, c.lat_d, c.ctrl_d, c.cnt,
                      c.blk_d, c.nblk, c.chunk, c.xcd, colour, c.sweep_base, k, c.X, c.Y, c.Z, c.PT, c.SX, c.SY, c.SZ,
                      c.surf_d, c.spart, fin, c.fused ? c.nlat : 0, c.sr_d, c.mom_d, c.obs_d);
@@ -1859,7 +1871,10 @@ int create_ctx(const mcl_lattice_desc *lat, int nlat, Ctx *c) {
     if (2ll * c->ld[r].nc > (c->flip ? kResMaxFlip : kResMax)) c->resident_ok = false;
     if (c->ld[r].nc > kResBlock) c->resident_one = false;
     if (c->ld[r].nx < 2) c->fused = false;
-    if ((long long)c->ld[r].nvalid != (long long)c->ld[r].nx * c->ld[r].ny * c->ld[r].nz) c->fused = false;
+    if ((long long)c->ld[r].nvalid != (long long)c->ld[r].nx * c->ld[r].ny * c->ld[r].nz) {
+      c->fused = false;
+      c->has_empty = true;
+    }
     if (c->ld[r].nx % 128 != 0) c->row = false;
   }
   // default launch shape (before any autotuning): 512 threads in dispatch order for a single lattice, 256 threads
