@@ -888,7 +888,8 @@ constexpr int kResChunk = 16384;  // sweeps per resident launch
 // ONE (no flip trial, every lattice at most kResBlock positions per colour): each thread takes one position and
 // computes the draw of its pair (p, p + 64) itself, the half k_sweep gives that position, so no lane runs two
 // trials in a row and twice the waves hide the latency (a separate instance: both ways in one kernel spill).
-template <int POT, bool MASK, bool SURF, bool FIELD, bool FLIP, bool FREE, bool ONE>
+// EMPTY = false: as k_sweep's (no lattice of the context has an empty site; the neighbours' point types are not read).
+template <int POT, bool MASK, bool SURF, bool FIELD, bool FLIP, bool FREE, bool ONE, bool EMPTY = true>
 __global__ __launch_bounds__(kResBlock) void k_sweep_resident(
     const LatDev *__restrict__ lats, CtrlDev *__restrict__ ctrl, unsigned int *__restrict__ cnt,
     const unsigned long long *__restrict__ sweep_base, unsigned k0, int nsw, float *__restrict__ X,
@@ -941,7 +942,7 @@ __global__ __launch_bounds__(kResBlock) void k_sweep_resident(
         Site s;
         uint32_t self = 0;
         if (t < L.nc &&
-            gather<MASK, SURF, FREE, false, POT != MCL_POT_LL>(L, colour, t, LX, LY, LZ, PTl, SXl, SYl, SZl, s, self)) {
+            gather<MASK, SURF, FREE, false, EMPTY && POT != MCL_POT_LL>(L, colour, t, LX, LY, LZ, PTl, SXl, SYl, SZl, s, self)) {
           const uint32_t ctr0 = (uint32_t)((t >> 7) * 64 + (t & 63));
           const mcl::Philox4 R = mcl::philox4x32_10(ctr0, (uint32_t)sweep, (uint32_t)(sweep >> 32),
                                                     kTagSweep + (uint32_t)colour, L.key0, L.key1);
@@ -970,7 +971,7 @@ __global__ __launch_bounds__(kResBlock) void k_sweep_resident(
             Site s;
             uint32_t self = 0;
             if (t >= L.nc ||
-                !gather<MASK, SURF, FREE, false, POT != MCL_POT_LL>(L, colour, t, LX, LY, LZ, PTl, SXl, SYl, SZl, s,
+                !gather<MASK, SURF, FREE, false, EMPTY && POT != MCL_POT_LL>(L, colour, t, LX, LY, LZ, PTl, SXl, SYl, SZl, s,
                                                                      self))
               continue;
             V3 nc;
@@ -1501,19 +1502,29 @@ void launch_sweep(dim3 g, dim3 b, hipStream_t st, const Ctx &c, int colour, unsi
 // folded into the previous sweep.
 dim3 sweep_grid(const Ctx *c) { return dim3(c->xcd ? 8 * c->chunk : c->nblk + (c->fused ? c->nlat : 0)); }
 using ResFn = void (*)(hipStream_t, const Ctx &, unsigned, int);
+template <int POT, bool MASK, bool SURF, bool FIELD, bool FLIP, bool FREE, bool ONE>
+void launch_resident_k(hipStream_t st, const Ctx &c, unsigned k0, int nsw) {
+  if constexpr (MASK && POT != MCL_POT_LL) {
+    if (!c.has_empty) {
+      hipLaunchKernelGGL((k_sweep_resident<POT, MASK, SURF, FIELD, FLIP, FREE, ONE, false>), dim3(c.nlat),
+                         dim3(kResBlock), 0, st, c.lat_d, c.ctrl_d, c.cnt, c.sweep_base, k0, nsw, c.X, c.Y, c.Z,
+                         c.PT, c.SX, c.SY, c.SZ, c.surf_d);
+      return;
+    }
+  }
+  hipLaunchKernelGGL((k_sweep_resident<POT, MASK, SURF, FIELD, FLIP, FREE, ONE>), dim3(c.nlat), dim3(kResBlock), 0,
+                     st, c.lat_d, c.ctrl_d, c.cnt, c.sweep_base, k0, nsw, c.X, c.Y, c.Z, c.PT, c.SX, c.SY, c.SZ,
+                     c.surf_d);
+}
 template <int POT, bool MASK, bool SURF, bool FIELD, bool FLIP, bool FREE>
 void launch_resident(hipStream_t st, const Ctx &c, unsigned k0, int nsw) {
   if constexpr (!FLIP) {
     if (c.resident_one) {
-      hipLaunchKernelGGL((k_sweep_resident<POT, MASK, SURF, FIELD, FLIP, FREE, true>), dim3(c.nlat), dim3(kResBlock),
-                         0, st, c.lat_d, c.ctrl_d, c.cnt, c.sweep_base, k0, nsw, c.X, c.Y, c.Z, c.PT, c.SX, c.SY,
-                         c.SZ, c.surf_d);
+      launch_resident_k<POT, MASK, SURF, FIELD, FLIP, FREE, true>(st, c, k0, nsw);
       return;
     }
   }
-  hipLaunchKernelGGL((k_sweep_resident<POT, MASK, SURF, FIELD, FLIP, FREE, false>), dim3(c.nlat), dim3(kResBlock), 0,
-                     st, c.lat_d, c.ctrl_d, c.cnt, c.sweep_base, k0, nsw, c.X, c.Y, c.Z, c.PT, c.SX, c.SY, c.SZ,
-                     c.surf_d);
+  launch_resident_k<POT, MASK, SURF, FIELD, FLIP, FREE, false>(st, c, k0, nsw);
 }
 template <int POT, bool MASK, bool SURF, bool FREE>
 ResFn pick_resident_field(bool field, bool flip) {
