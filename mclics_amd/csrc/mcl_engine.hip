@@ -634,10 +634,12 @@ __device__ __forceinline__ bool metropolis_trial(const LatDev &L, const Site &s,
                                                  bool &changed) {
   float u_ang, rot, u;
   if constexpr (SPT == 2) {
-    const uint32_t w = R.v[2 * e];
+    // selects, not R.v[2 * e]: a run-time e would index the private array, which the compiler then moves to LDS
+    // (16 B per thread of the 1024-thread bound) and reads back there
+    const uint32_t w = e ? R.v[2] : R.v[0];
     u_ang = mcl::u01_open16(w >> 16);
     rot = mcl::u01_open16(w & 0xFFFFu);
-    u = mcl::u01_open(R.v[2 * e + 1]);
+    u = mcl::u01_open(e ? R.v[3] : R.v[1]);
   } else {
     u_ang = mcl::u01_open(R.v[0]);
     rot = mcl::u01_open(R.v[1]);
@@ -683,7 +685,11 @@ __device__ __forceinline__ bool metropolis_trial(const LatDev &L, const Site &s,
 
 // EMPTY = false (masked GHRL and pear contexts none of whose lattices has an empty site, such as the slab): the
 // neighbours' point types are not loaded, since none of them can be 0 (LL never loads them, see gather's NBPT).
-template <int POT, bool MASK, bool SURF, bool FIELD, bool FLIP, bool FREE, bool MEAS, bool ROW, bool EMPTY = true>
+// SPLIT (small contexts without the flip trial, not row-aligned): each thread takes one position instead of a pair,
+// computes the pair's Philox draw itself and uses the half the pair's thread would give that position, so the
+// trials are the same and a wave's serial work halves (few waves per SIMD; the 32^3 and 64^3 lattices).
+template <int POT, bool MASK, bool SURF, bool FIELD, bool FLIP, bool FREE, bool MEAS, bool ROW, bool EMPTY = true,
+          bool SPLIT = false>
 // Launch shape (chosen by the autotuner): any multiple of 64 up to 1024 threads per workgroup; with xcd != 0
 // workgroup b runs logical tile (b & 7) * chunk + (b >> 3), so consecutive tiles share an XCD and its L2.
 // The sweep number is *sweep_base + k: graphs replay fixed arguments while the base advances on the device.
@@ -725,7 +731,8 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(MEAS && (M
   const unsigned long long sweep = *sweep_base + k;
   const float T = ctrl[r].T, invT = ctrl[r].invT, elecE = ctrl[r].elecE;
   const int lane = threadIdx.x & 63;
-  const int lblk = bd.y / ((FLIP ? 1 : 2) * (int)blockDim.x);  // workgroup index within the lattice
+  static_assert(!SPLIT || (!FLIP && !ROW), "SPLIT is for pair-sharing generic passes");
+  const int lblk = bd.y / ((FLIP || SPLIT ? 1 : 2) * (int)blockDim.x);  // workgroup index within the lattice
   unsigned int *const cnt_r = cnt + (size_t)r * kCntLattice;
   // Sites per thread: without the flip trial a pair of positions (p, p + 64) of the colour block shares one
   // Philox4x32-10 draw, 64 bits per site (16-bit angle, 16-bit axis choice, 32-bit acceptance uniform): each wave
@@ -734,8 +741,10 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(MEAS && (M
   // tag + colour) depends on the position only, never on the launch shape (a workgroup spans a multiple of 128
   // positions).
   constexpr int SPT = FLIP ? 1 : 2;
-  const int tw = SPT == 2 ? bd.y + 128 * (int)(threadIdx.x >> 6) + lane : bd.y + (int)threadIdx.x;
-  const uint32_t ctr0 = SPT == 2 ? (uint32_t)(((bd.y >> 7) + (int)(threadIdx.x >> 6)) * 64 + lane) : (uint32_t)tw;
+  const int tw = SPT == 2 && !SPLIT ? bd.y + 128 * (int)(threadIdx.x >> 6) + lane : bd.y + (int)threadIdx.x;
+  const uint32_t ctr0 = SPLIT ? (uint32_t)((tw >> 7) * 64 + (tw & 63))
+                              : (SPT == 2 ? (uint32_t)(((bd.y >> 7) + (int)(threadIdx.x >> 6)) * 64 + lane)
+                                          : (uint32_t)tw);
   const mcl::Philox4 R = mcl::philox4x32_10(ctr0, (uint32_t)sweep, (uint32_t)(sweep >> 32),
                                             kTagSweep + (uint32_t)colour, L.key0, L.key1);
   float av;
@@ -772,7 +781,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(MEAS && (M
     qm[6] += 1.0;
   };
 #pragma unroll
-  for (int e = 0; e < SPT; e++) {
+  for (int e = 0; e < (SPLIT ? 1 : SPT); e++) {
     Site s;
     uint32_t self = 0;
     if constexpr (ROW) {
@@ -784,7 +793,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(MEAS && (M
                                                                                    SX, SY, SZ, s, self))
         continue;
     } else {
-      const int t = tw + 64 * e;
+      const int t = SPLIT ? tw : tw + 64 * e;
       if (t >= L.nc || !gather<MASK, SURF, FREE, MEAS && MASK, EMPTY && POT != MCL_POT_LL>(L, colour, t, X, Y, Z, PT,
                                                                                           SX, SY, SZ, s, self))
         continue;
@@ -792,7 +801,8 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(MEAS && (M
     V3 nc;
     float E_fin;
     bool changed;
-    nacc += metropolis_trial<POT, SURF, FIELD, FLIP, SPT>(L, s, R, e, av, T, invT, elecE, surf, nc, E_fin, changed)
+    nacc += metropolis_trial<POT, SURF, FIELD, FLIP, SPT>(L, s, R, SPLIT ? (tw >> 6) & 1 : e, av, T, invT, elecE,
+                                                          surf, nc, E_fin, changed)
                 ? 1u
                 : 0u;
     if (changed) {
@@ -1449,6 +1459,7 @@ struct Ctx {
   // measurement folded into the colour-1 sweep pass (unmasked lattices without the radial or pear observables)
   bool fused = false;
   bool has_empty = false;  // some lattice has an empty site (masked GHRL / pear sweeps then check neighbours)
+  bool split = false;      // one position per sweep thread (SPLIT): a small, not row-aligned context without flips
   double *spart = nullptr;  // kNQS partials per workgroup of the sweep work list
   size_t spart_cap = 0;     // workgroups spart holds
   int2 *sr_d = nullptr;     // per lattice: its workgroups [first, last) in the sweep work list
@@ -1466,20 +1477,26 @@ struct Ctx {
 using SweepFn = void (*)(dim3, dim3, hipStream_t, const Ctx &, int, unsigned, bool, int);
 using MeasFn = void (*)(dim3, dim3, hipStream_t, const Ctx &);
 
-template <int POT, bool MASK, bool SURF, bool FIELD, bool FLIP, bool FREE, bool MEAS, bool ROW>
+template <int POT, bool MASK, bool SURF, bool FIELD, bool FLIP, bool FREE, bool MEAS, bool ROW, bool SPLIT = false>
 void launch_sweep_k(dim3 g, dim3 b, hipStream_t st, const Ctx &c, int colour, unsigned k, int fin) {
-  if constexpr (MASK && POT != MCL_POT_LL) {
-    if (!c.has_empty) {
-      hipLaunchKernelGGL((k_sweep<POT, MASK, SURF, FIELD, FLIP, FREE, MEAS, ROW, false>), g, b, 0, st, c.lat_d,
-                         c.ctrl_d, c.cnt, c.blk_d, c.nblk, c.chunk, c.xcd, colour, c.sweep_base, k, c.X, c.Y, c.Z,
-                         c.PT, c.SX, c.SY, c.SZ, c.surf_d, c.spart, fin, c.fused ? c.nlat : 0, c.sr_d, c.mom_d,
-                         c.obs_d);
+  if constexpr (!FLIP && !ROW && !SPLIT) {
+    if (c.split) {
+      launch_sweep_k<POT, MASK, SURF, FIELD, FLIP, FREE, MEAS, ROW, true>(g, b, st, c, colour, k, fin);
       return;
     }
   }
-  hipLaunchKernelGGL((k_sweep<POT, MASK, SURF, FIELD, FLIP, FREE, MEAS, ROW>), g, b, 0, st, c.lat_d, c.ctrl_d, c.cnt,
-                     c.blk_d, c.nblk, c.chunk, c.xcd, colour, c.sweep_base, k, c.X, c.Y, c.Z, c.PT, c.SX, c.SY, c.SZ,
-                     c.surf_d, c.spart, fin, c.fused ? c.nlat : 0, c.sr_d, c.mom_d, c.obs_d);
+  if constexpr (MASK && POT != MCL_POT_LL) {
+    if (!c.has_empty) {
+      hipLaunchKernelGGL((k_sweep<POT, MASK, SURF, FIELD, FLIP, FREE, MEAS, ROW, false, SPLIT>), g, b, 0, st,
+                         c.lat_d, c.ctrl_d, c.cnt, c.blk_d, c.nblk, c.chunk, c.xcd, colour, c.sweep_base, k, c.X,
+                         c.Y, c.Z, c.PT, c.SX, c.SY, c.SZ, c.surf_d, c.spart, fin, c.fused ? c.nlat : 0, c.sr_d,
+                         c.mom_d, c.obs_d);
+      return;
+    }
+  }
+  hipLaunchKernelGGL((k_sweep<POT, MASK, SURF, FIELD, FLIP, FREE, MEAS, ROW, true, SPLIT>), g, b, 0, st, c.lat_d,
+                     c.ctrl_d, c.cnt, c.blk_d, c.nblk, c.chunk, c.xcd, colour, c.sweep_base, k, c.X, c.Y, c.Z, c.PT,
+                     c.SX, c.SY, c.SZ, c.surf_d, c.spart, fin, c.fused ? c.nlat : 0, c.sr_d, c.mom_d, c.obs_d);
 }
 
 template <int POT, bool MASK, bool SURF, bool FIELD, bool FLIP, bool FREE>
@@ -1678,7 +1695,7 @@ int dev_alloc(T **p, size_t n) {
 // positions. Drops the captured graphs, which hold the old launch shape.
 int build_worklist(Ctx *c, int bs, int xcd) {
   // a workgroup of the sweep covers bs colour-block positions, or 2 bs when a thread updates a pair of sites
-  const int span = c->flip ? bs : 2 * bs;
+  const int span = c->flip || c->split ? bs : 2 * bs;
   std::vector<int2> blk, sr(c->nlat);
   for (int r = 0; r < c->nlat; r++) {
     sr[r].x = (int)blk.size();
@@ -1888,13 +1905,18 @@ int create_ctx(const mcl_lattice_desc *lat, int nlat, Ctx *c) {
     }
     if (c->ld[r].nx % 128 != 0) c->row = false;
   }
+  {
+    long long npos = 0;
+    for (int r = 0; r < nlat; r++) npos += c->ld[r].nc;
+    c->split = !c->flip && !c->row && npos <= (1ll << 17);
+  }
   // default launch shape (before any autotuning): 512 threads in dispatch order for a single lattice, 256 threads
   // in XCD-contiguous tiles for a batch, the faster shapes measured for the 128^3 benchmark and the 512-lattice batch.
   // A single lattice too small for 1024 workgroups of 512 threads takes smaller workgroups (down to 64 threads), so
   // that its work spreads over more of the 256 CUs. Deterministic: the same lattice always gets the same shape.
   int bs0 = 256;
   if (nlat == 1) {
-    const long long span = c->flip ? 1 : 2;
+    const long long span = c->flip || c->split ? 1 : 2;
     bs0 = 512;
     while (bs0 > 64 && (long long)c->ld[0].nc < 1024ll * span * bs0) bs0 >>= 1;
   }
