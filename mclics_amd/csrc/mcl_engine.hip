@@ -687,7 +687,7 @@ __device__ __forceinline__ bool metropolis_trial(const LatDev &L, const Site &s,
 // neighbours' point types are not loaded, since none of them can be 0 (LL never loads them, see gather's NBPT).
 // SPLIT (small contexts without the flip trial, not row-aligned): each thread takes one position instead of a pair,
 // computes the pair's Philox draw itself and uses the half the pair's thread would give that position, so the
-// trials are the same and a wave's serial work halves (few waves per SIMD; the 32^3 and 64^3 lattices).
+// trials are the same and a wave's serial work halves (few waves per SIMD; LL lattices up to 80^3).
 template <int POT, bool MASK, bool SURF, bool FIELD, bool FLIP, bool FREE, bool MEAS, bool ROW, bool EMPTY = true,
           bool SPLIT = false>
 // Launch shape (chosen by the autotuner): any multiple of 64 up to 1024 threads per workgroup; with xcd != 0
@@ -1908,7 +1908,7 @@ int create_ctx(const mcl_lattice_desc *lat, int nlat, Ctx *c) {
   {
     long long npos = 0;
     for (int r = 0; r < nlat; r++) npos += c->ld[r].nc;
-    c->split = !c->flip && !c->row && npos <= (1ll << 17);
+    c->split = !c->flip && !c->row && npos <= (1ll << 18);
   }
   // default launch shape (before any autotuning): 512 threads in dispatch order for a single lattice, 256 threads
   // in XCD-contiguous tiles for a batch, the faster shapes measured for the 128^3 benchmark and the 512-lattice batch.

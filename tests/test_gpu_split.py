@@ -1,4 +1,4 @@
-"""One position per thread (the SPLIT sweep instance) for small contexts: a context of at most 2^17 colour-block
+"""One position per thread (the SPLIT sweep instance) for small contexts: a context of at most 2^18 colour-block
 positions without the flip trial and not row-aligned runs it. The draws and trials are those of the pair path, so a
 lattice swept alone (split) must follow the trajectory it follows inside a larger context (pairs) bit for bit."""
 import numpy as np
@@ -43,8 +43,8 @@ def _run(capi, lats):
 def test_split_sweeps_follow_the_pair_path(mcl, name):
     print("split case", name, flush=True)
     lat = _lattice(mcl, name)
-    # a 96 x 96 x 32 partner makes the context too large for the split instance (147,456 positions per colour)
-    big = mcl.Lattice(96, 96, 32, batch.random_directors(np.ones(96 * 96 * 32, dtype=np.int32), 3),
+    # a 96 x 96 x 64 partner makes the context too large for the split instance (294,912 positions per colour)
+    big = mcl.Lattice(96, 96, 64, batch.random_directors(np.ones(96 * 96 * 64, dtype=np.int32), 3),
                       model=lat.model, T=1.0, seed=9)
     da, ava, sa, ma = _run(mcl, lat)
     db, avb, sb, mb = _run(mcl, [lat, big])
