@@ -22,6 +22,10 @@ def _lattice(capi, name):
         pt, ns = batch.sphere_geometry(32)
         return capi.Lattice(32, 32, 32, batch.random_directors(pt, 5), periodic=(0, 0, 0), pt=pt, ns=ns,
                             surfaces=[("homeotropic", 1.0, (0, 0, 0))], model=capi.make_model("ll"), T=0.9, seed=6)
+    if name == "ll_odd_free_33x30x20":
+        # odd nx: the colour blocks carry padding positions; free x and z, a field
+        return capi.Lattice(33, 30, 20, batch.random_directors(ones(33 * 30 * 20), 11), periodic=(0, 1, 0),
+                            model=capi.make_model("ll", elec_dir=(0.0, 0.6, 0.8), elecA=0.4), T=0.7, seed=12)
     assert name == "ghrl_slab_24x24x20"
     pt, ns = batch.slab_geometry(24, 24, 20)
     return capi.Lattice(24, 24, 20, batch.random_directors(pt, 7), periodic=(1, 1, 0), pt=pt, ns=ns,
@@ -39,7 +43,7 @@ def _run(capi, lats):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("name", ["ll_32", "ll_64", "ll_sphere_32", "ghrl_slab_24x24x20"])
+@pytest.mark.parametrize("name", ["ll_32", "ll_64", "ll_sphere_32", "ll_odd_free_33x30x20", "ghrl_slab_24x24x20"])
 def test_split_sweeps_follow_the_pair_path(mcl, name):
     print("split case", name, flush=True)
     lat = _lattice(mcl, name)
