@@ -545,6 +545,25 @@ __device__ __forceinline__ double wave_sum(double v) {
   return (readlane_f64(v, 0) + readlane_f64(v, 16)) + (readlane_f64(v, 32) + readlane_f64(v, 48));
 }
 
+// Row sums of 8 doubles at once by reduce-scatter, in a fixed order. Each step pairs lanes that hold the same
+// subset of the quantities; each lane keeps half of them and adds its partner's copies: row_half_mirror (partner
+// l ^ 7, lane bit 2 picks the half), quad_perm [1,0,3,2] (l ^ 1, bit 0), quad_perm [2,3,0,1] (l ^ 2, bit 1), then
+// row_ror 8 (l ^ 8) adds the other half-row. Lane l ends with the sum over its row of 16 lanes of quantity
+// rs8_index(l); lanes l and l ^ 8 hold the same value. About 50 VALU instructions for the 8 quantities, against 8
+// wave_sum calls of about 23 each. All lanes of the wave must be active.
+__device__ __forceinline__ int rs8_index(int lane) { return (lane & 4) + ((lane & 1) << 1) + ((lane & 2) >> 1); }
+
+__device__ __forceinline__ double row_rs8(const double q[8], int lane) {
+  const bool b2 = lane & 4, b0 = lane & 1, b1 = lane & 2;
+  double r[4], s[2];
+#pragma unroll
+  for (int i = 0; i < 4; i++) r[i] = (b2 ? q[i + 4] : q[i]) + dpp_f64<0x141>(b2 ? q[i] : q[i + 4]);
+#pragma unroll
+  for (int j = 0; j < 2; j++) s[j] = (b0 ? r[j + 2] : r[j]) + dpp_f64<0xB1>(b0 ? r[j] : r[j + 2]);
+  const double t = (b1 ? s[1] : s[0]) + dpp_f64<0x4E>(b1 ? s[0] : s[1]);
+  return t + dpp_f64<0x128>(t);
+}
+
 // ang_var rule of evolve.cu:430-436. The reference compares the double quotient acc / nvalid with 0.5; for
 // integers below 2^31 that is the same as comparing 2 acc with nvalid (the quotient is never within rounding of
 // 0.5 unless it equals 0.5), which avoids an FP64 division. The updates keep the reference's float * double
@@ -861,7 +880,22 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(MEAS && (M
     for (int w = 0; w < nw; w++) tot += wsum[w];
     if (tot) atomicAdd(&cnt_r[(sweep % 3) * kCntSweep + (lblk & (kCnt - 1)) * kCntStride], tot);
   }
-  if constexpr (MEAS) {
+  if constexpr (MEAS && (MASK || POT != MCL_POT_LL)) {
+    static_assert(kNQS == 8, "row_rs8 reduces 8 quantities");
+    // per wave and row of 16 lanes, the row sums of the 8 quantities (lanes 0-7 of each row write them). Only in
+    // the instances that run at 6 waves per SIMD: under the 8-wave bound of the others the reduce-scatter spills.
+    __shared__ double mred[16][4][kNQS];
+    const double x = row_rs8(qm, lane);
+    if ((lane & 8) == 0) mred[threadIdx.x >> 6][lane >> 4][rs8_index(lane)] = x;
+    __syncthreads();
+    if (threadIdx.x < kNQS) {
+      double v = 0.0;
+      const int nw = (int)(blockDim.x >> 6);
+      for (int w = 0; w < nw; w++)
+        v += (mred[w][0][threadIdx.x] + mred[w][1][threadIdx.x]) + (mred[w][2][threadIdx.x] + mred[w][3][threadIdx.x]);
+      spart[(size_t)threadIdx.x * nblk + Lb] = v;
+    }
+  } else if constexpr (MEAS) {
     __shared__ double mred[16][kNQS];
 #pragma unroll
     for (int a = 0; a < kNQS; a++) {
