@@ -9,32 +9,18 @@
 #include <strings.h>
 
 #include <algorithm>
+#include <cmath>
 #include <fstream>
 #include <string>
 #include <vector>
 
+#include "coltable.h"
 #include "philox.h"
 
 namespace {
 
 inline size_t lin(const Lattice &L, int i, int j, int k) {
   return (size_t)i + (size_t)L.Nx * ((size_t)j + (size_t)L.Ny * k);
-}
-
-// Splits one line of a comma-separated file into trimmed fields (io.cpp:92-105).
-std::vector<std::string> split_csv(const std::string &line) {
-  std::vector<std::string> out;
-  std::string field;
-  for (size_t k = 0; k <= line.size(); k++) {
-    if (k == line.size() || line[k] == ',') {
-      size_t a = field.find_first_not_of(" \t\r"), b = field.find_last_not_of(" \t\r");
-      out.push_back(a == std::string::npos ? "" : field.substr(a, b - a + 1));
-      field.clear();
-    } else {
-      field += line[k];
-    }
-  }
-  return out;
 }
 
 int boundary_kind(const std::string &type, const char *axis) {
@@ -84,101 +70,94 @@ void geometry_sphere(Lattice &L) {
       }
 }
 
-void geometry_error(const char *file, int line, const char *msg) {
-  fprintf(stderr, "  [ERROR] %s, line %d: %s\n", file, line, msg);
-  exit(10);
+// A record's grid coordinate (or point type) is usable when it is a whole number that fits an int.
+bool whole(double v) { return v == std::trunc(v) && std::fabs(v) < 2147483648.0; }
+
+// Sites named by the records of a table: each must lie in the grid and be named at most once.
+class SiteRoll {
+ public:
+  explicit SiteRoll(const Lattice &L) : L_(L), named_(L.pt.size(), 0) {}
+
+  // Site of the whole coordinates r[0..2], or a table error when it is outside the grid.
+  size_t locate(const ColumnTable &t, const double *r) const {
+    const int i = (int)r[0], j = (int)r[1], k = (int)r[2];
+    if (i < 0 || i >= L_.Nx || j < 0 || j >= L_.Ny || k < 0 || k >= L_.Nz)
+      t.fail("point (%d, %d, %d) outside the %d x %d x %d grid.", i, j, k, L_.Nx, L_.Ny, L_.Nz);
+    return lin(L_, i, j, k);
+  }
+
+  // Marks site s as named, or a table error when a record named it before.
+  void take(const ColumnTable &t, size_t s) {
+    if (named_[s]) {
+      const long nx = L_.Nx, ny = L_.Ny;
+      t.fail("point (%ld, %ld, %ld) is listed twice.", (long)s % nx, (long)(s / nx) % ny, (long)(s / nx / ny));
+    }
+    named_[s] = 1;
+    count_++;
+  }
+
+  bool named(size_t s) const { return named_[s] != 0; }
+  size_t count() const { return count_; }
+
+ private:
+  const Lattice &L_;
+  std::vector<char> named_;
+  size_t count_ = 0;
+};
+
+// Rescales (x, y, z) to unit length when it is clearly off (|length - 1| > 1e-3), so that files written with rounded
+// digits are taken as they are. Returns whether it rescaled.
+bool rescale_if_off(double &x, double &y, double &z) {
+  const double len = std::sqrt(x * x + y * y + z * z);
+  if (std::fabs(len - 1.0) <= 1e-3) return false;
+  x /= len;
+  y /= len;
+  z /= len;
+  return true;
 }
 
-// Custom geometry file: header naming the columns x, y, z, nx, ny, nz, pt (any order, other columns ignored),
-// then one line per listed point; unlisted points are bulk (geometry_custom.cpp:62-163).
-int geometry_custom(Lattice &L, const std::string &fname) {
-  std::ifstream in(fname);
-  if (!in) {
-    perror(fname.c_str());
-    exit(2);
-  }
-  printf("  Boundary file: %s\n", fname.c_str());
-  const char *f = fname.c_str();
-  std::string line;
-  if (!std::getline(in, line)) geometry_error(f, 1, "empty file (a header line is expected).");
-  std::vector<std::string> header = split_csv(line);
-  const char *names[7] = {"x", "y", "z", "nx", "ny", "nz", "pt"};
-  int col[7];
-  std::string missing;
-  for (int c = 0; c < 7; c++) {
-    col[c] = -1;
-    for (size_t h = 0; h < header.size(); h++)
-      if (strcasecmp(header[h].c_str(), names[c]) == 0) col[c] = (int)h;
-    if (col[c] < 0) missing += std::string(missing.empty() ? "" : ", ") + names[c];
-  }
-  if (!missing.empty()) {
-    std::string msg = "missing column(s) in the header: " + missing + " (required: x,y,z,nx,ny,nz,pt).";
-    geometry_error(f, 1, msg.c_str());
-  }
-  printf("  Column positions: x=%d y=%d z=%d nx=%d ny=%d nz=%d pt=%d\n", col[0], col[1], col[2], col[3], col[4],
-         col[5], col[6]);
-  const size_t Nt = L.pt.size();
-  std::vector<char> seen(Nt, 0);
+// Custom geometry (behaviour of geometry_custom.cpp:62-163): a table with the columns x, y, z, nx, ny, nz, pt. Each
+// record sets one site's point type (0 empty, 1 bulk, n >= 2 surface n - 2) and, on surface sites, its outward
+// normal; sites without a record are bulk. Format errors end the program with status 10. Returns the number of
+// surfaces, the largest point type less one.
+int geometry_custom(Lattice &L, const std::string &path) {
+  ColumnTable table(path, 10);
+  printf("  Boundary file: %s\n", path.c_str());
+  table.bind({"x", "y", "z", "nx", "ny", "nz", "pt"}, "x,y,z,nx,ny,nz,pt");
+  printf("  Column positions: x=%d y=%d z=%d nx=%d ny=%d nz=%d pt=%d\n", table.position(0), table.position(1),
+         table.position(2), table.position(3), table.position(4), table.position(5), table.position(6));
   std::fill(L.pt.begin(), L.pt.end(), 1);
-  int max_kind = 0, line_number = 1, renormalized = 0;
-  size_t listed = 0;
-  char msg[300];
-  while (std::getline(in, line)) {
-    line_number++;
-    if (line.find_first_not_of(" \t\r") == std::string::npos) continue;
-    std::vector<std::string> fl = split_csv(line);
-    if (fl.size() != header.size()) {
-      snprintf(msg, sizeof(msg), "%zu fields, but the header has %zu.", fl.size(), header.size());
-      geometry_error(f, line_number, msg);
-    }
-    double v[7];
-    for (int c = 0; c < 7; c++) {
-      char *end;
-      v[c] = strtod(fl[col[c]].c_str(), &end);
-      if (fl[col[c]].empty() || *end != '\0') {
-        snprintf(msg, sizeof(msg), "column %s is not a number: '%s'.", names[c], fl[col[c]].c_str());
-        geometry_error(f, line_number, msg);
-      }
-    }
-    const int ii = (int)v[0], jj = (int)v[1], kk = (int)v[2], type = (int)v[6];
-    if (v[0] != ii || v[1] != jj || v[2] != kk || v[6] != type)
-      geometry_error(f, line_number, "x, y, z and pt must be integers.");
-    if (ii < 0 || ii >= L.Nx || jj < 0 || jj >= L.Ny || kk < 0 || kk >= L.Nz) {
-      snprintf(msg, sizeof(msg), "point (%d, %d, %d) outside the %d x %d x %d grid.", ii, jj, kk, L.Nx, L.Ny, L.Nz);
-      geometry_error(f, line_number, msg);
-    }
-    if (type < 0) geometry_error(f, line_number, "negative point type (use 0 empty, 1 bulk, 2+ surface).");
-    const size_t idx = lin(L, ii, jj, kk);
-    if (seen[idx]) {
-      snprintf(msg, sizeof(msg), "point (%d, %d, %d) is listed twice.", ii, jj, kk);
-      geometry_error(f, line_number, msg);
-    }
-    seen[idx] = 1;
-    listed++;
-    double nx = v[3], ny = v[4], nz = v[5], norm = sqrt(nx * nx + ny * ny + nz * nz);
+  SiteRoll roll(L);
+  int top_type = 0, rescaled = 0;
+  double r[7];
+  while (table.next(r)) {
+    if (!whole(r[0]) || !whole(r[1]) || !whole(r[2]) || !whole(r[6]))
+      table.fail("x, y, z and pt must be integers.");
+    const size_t s = roll.locate(table, r);
+    const int type = (int)r[6];
+    if (type < 0) table.fail("negative point type (use 0 empty, 1 bulk, 2+ surface).");
+    roll.take(table, s);
+    double nx = r[3], ny = r[4], nz = r[5];
     if (type >= 2) {
-      if (norm < 1e-6) geometry_error(f, line_number, "surface point with a zero normal vector.");
-      if (fabs(norm - 1) > 1e-3) {
-        renormalized++;
-        nx /= norm;
-        ny /= norm;
-        nz /= norm;
-      }
+      if (std::sqrt(nx * nx + ny * ny + nz * nz) < 1e-6) table.fail("surface point with a zero normal vector.");
+      if (rescale_if_off(nx, ny, nz)) rescaled++;
     }
-    L.ns[3 * idx + 0] = (float)nx;
-    L.ns[3 * idx + 1] = (float)ny;
-    L.ns[3 * idx + 2] = (float)nz;
-    L.pt[idx] = type;
-    max_kind = std::max(max_kind, type);
+    L.pt[s] = type;
+    L.ns[3 * s + 0] = (float)nx;
+    L.ns[3 * s + 1] = (float)ny;
+    L.ns[3 * s + 2] = (float)nz;
+    top_type = std::max(top_type, type);
   }
-  if (listed < Nt)
+  const size_t Nt = L.pt.size();
+  if (roll.count() < Nt)
     fprintf(stderr, "  [WARNING] %zu of the %zu grid points are not listed in %s; they were set to bulk (pt = 1).\n",
-            Nt - listed, Nt, f);
-  if (renormalized > 0)
+            Nt - roll.count(), Nt, path.c_str());
+  if (rescaled > 0)
     fprintf(stderr, "  [WARNING] %d surface normal(s) in %s did not have unit length and were normalized.\n",
-            renormalized, f);
-  const int nsurf = max_kind > 1 ? max_kind - 1 : 0;
+            rescaled, path.c_str());
+  const int nsurf = top_type > 1 ? top_type - 1 : 0;
   printf("  %d boundary surface(s) found.\n\n", nsurf);
+  fflush(stdout);
   return nsurf;
 }
 
@@ -313,126 +292,67 @@ void ic_homogeneous(Lattice &L, float theta, float phi) {
   }
 }
 
+// Cholesteric helix along z (behaviour of ic.cpp:202-232): polar angle theta_0, azimuth phi_0 - 2 pi k / p0 on plane
+// k (p0_i replaces p0 when it is set). Every site of a plane gets the plane's director; empty sites get (0, 0, 1).
+// The angle steps keep the reference's precisions: the azimuth is rounded to float, the components are computed in
+// double from it and stored as float.
 void ic_cholesteric(Lattice &L, const Params &p) {
-  const float theta = p.theta_0;
-  const float phi_0 = (float)(p.phi_0 * M_PI / 180);
-  const float p0 = p.p0_i ? p.p0_i : p.p0;
+  const float pitch = p.p0_i ? p.p0_i : p.p0;
   printf("  Initial condition: cholesteric\n");
-  printf("  %-10s %g\n", "p0:", p0);
+  printf("  %-10s %g\n", "p0:", pitch);
   printf("  %-10s %g\n", "phi0:", p.phi_0);
-  printf("  %-10s %g\n\n", "theta0:", theta);
-  for (int i = 0; i < L.Nx; i++)
-    for (int j = 0; j < L.Ny; j++)
-      for (int k = 0; k < L.Nz; k++) {
-        const size_t s = lin(L, i, j, k);
-        if (!L.pt[s]) {
-          set_empty(L, s);
-          continue;
-        }
-        const float phi = (float)(phi_0 - 2 * M_PI * k / p0);
-        L.ni[3 * s + 0] = (float)(sin(theta * M_PI / 180) * cos(phi));
-        L.ni[3 * s + 1] = (float)(sin(theta * M_PI / 180) * sin(phi));
-        L.ni[3 * s + 2] = (float)cos(theta * M_PI / 180);
-      }
-}
-
-void ic_error(const char *file, int line, const char *msg) {
-  fprintf(stderr, "  [ERROR] %s, line %d: %s\n", file, line, msg);
-  exit(3);
-}
-
-// Initial-condition file: header with x, y, z, nx, ny, nz (any order, other columns ignored), one line per point;
-// every non-empty site must be listed; non-unit directors (|n| off by more than 1e-3) are normalized
-// (ic.cpp:103-200).
-void ic_file(Lattice &L, const std::string &fname) {
-  const char *f = fname.c_str();
-  std::ifstream in(fname);
-  if (!in) {
-    perror(f);
-    exit(2);
+  printf("  %-10s %g\n\n", "theta0:", p.theta_0);
+  const double polar = p.theta_0 * M_PI / 180;
+  const float azimuth_0 = (float)(p.phi_0 * M_PI / 180);
+  const size_t plane = (size_t)L.Nx * L.Ny;
+  for (int k = 0; k < L.Nz; k++) {
+    const float azimuth = (float)(azimuth_0 - 2 * M_PI * k / pitch);
+    const float d[3] = {(float)(sin(polar) * cos(azimuth)), (float)(sin(polar) * sin(azimuth)), (float)cos(polar)};
+    for (size_t s = plane * k; s < plane * (k + 1); s++) {
+      if (L.pt[s])
+        std::copy(d, d + 3, &L.ni[3 * s]);
+      else
+        set_empty(L, s);
+    }
   }
-  printf("  Initial condition: file (%s)\n", f);
+}
+
+// Initial condition from a file (behaviour of ic.cpp:103-200): a table with the columns x, y, z, nx, ny, nz (a
+// director_field snapshot qualifies). Every non-empty site must have a record; directors off unit length by more
+// than 1e-3 are rescaled; empty sites get (0, 0, 1) whatever their record says. Format errors end the program with
+// status 3.
+void ic_file(Lattice &L, const std::string &path) {
+  ColumnTable table(path, 3);
+  printf("  Initial condition: file (%s)\n", path.c_str());
   fflush(stdout);
-  std::string line;
-  if (!std::getline(in, line)) ic_error(f, 1, "empty file (a header line is expected).");
-  std::vector<std::string> header = split_csv(line);
-  const char *names[6] = {"x", "y", "z", "nx", "ny", "nz"};
-  int col[6];
-  std::string missing;
-  for (int c = 0; c < 6; c++) {
-    col[c] = -1;
-    for (size_t h = 0; h < header.size(); h++)
-      if (strcasecmp(header[h].c_str(), names[c]) == 0) col[c] = (int)h;
-    if (col[c] < 0) missing += std::string(missing.empty() ? "" : ", ") + names[c];
-  }
-  if (!missing.empty()) {
-    std::string msg = "missing column(s) in the header: " + missing + " (required: x,y,z,nx,ny,nz).";
-    ic_error(f, 1, msg.c_str());
-  }
+  table.bind({"x", "y", "z", "nx", "ny", "nz"}, "x,y,z,nx,ny,nz");
   const size_t Nt = L.pt.size();
-  std::vector<char> seen(Nt, 0);
-  int line_number = 1, normalized = 0;
-  char msg[300];
-  while (std::getline(in, line)) {
-    line_number++;
-    if (line.find_first_not_of(" \t\r") == std::string::npos) continue;
-    std::vector<std::string> fl = split_csv(line);
-    if (fl.size() != header.size()) {
-      snprintf(msg, sizeof(msg), "%zu fields, but the header has %zu.", fl.size(), header.size());
-      ic_error(f, line_number, msg);
-    }
-    double v[6];
-    for (int c = 0; c < 6; c++) {
-      char *end;
-      v[c] = strtod(fl[col[c]].c_str(), &end);
-      if (fl[col[c]].empty() || *end != '\0') {
-        snprintf(msg, sizeof(msg), "column %s is not a number: '%s'.", names[c], fl[col[c]].c_str());
-        ic_error(f, line_number, msg);
-      }
-    }
-    const int i = (int)v[0], j = (int)v[1], k = (int)v[2];
-    if (v[0] != i || v[1] != j || v[2] != k) ic_error(f, line_number, "x, y and z must be integers.");
-    if (i < 0 || i >= L.Nx || j < 0 || j >= L.Ny || k < 0 || k >= L.Nz) {
-      snprintf(msg, sizeof(msg), "point (%d, %d, %d) outside the %d x %d x %d grid.", i, j, k, L.Nx, L.Ny, L.Nz);
-      ic_error(f, line_number, msg);
-    }
-    const size_t idx = lin(L, i, j, k);
-    if (seen[idx]) {
-      snprintf(msg, sizeof(msg), "point (%d, %d, %d) is listed twice.", i, j, k);
-      ic_error(f, line_number, msg);
-    }
-    seen[idx] = 1;
-    double nx = v[3], ny = v[4], nz = v[5], norm = sqrt(nx * nx + ny * ny + nz * nz);
-    if (L.pt[idx]) {
-      if (norm < 1e-6) ic_error(f, line_number, "zero director on a non-empty site.");
-      if (fabs(norm - 1) > 1e-3) {
-        normalized++;
-        nx /= norm;
-        ny /= norm;
-        nz /= norm;
-      }
-    } else {
-      nx = 0;
-      ny = 0;
-      nz = 1;
-    }
-    L.ni[3 * idx + 0] = (float)nx;
-    L.ni[3 * idx + 1] = (float)ny;
-    L.ni[3 * idx + 2] = (float)nz;
+  for (size_t s = 0; s < Nt; s++) set_empty(L, s);
+  SiteRoll roll(L);
+  int rescaled = 0;
+  double r[6];
+  while (table.next(r)) {
+    if (!whole(r[0]) || !whole(r[1]) || !whole(r[2])) table.fail("x, y and z must be integers.");
+    const size_t s = roll.locate(table, r);
+    roll.take(table, s);
+    if (!L.pt[s]) continue;
+    double nx = r[3], ny = r[4], nz = r[5];
+    if (std::sqrt(nx * nx + ny * ny + nz * nz) < 1e-6) table.fail("zero director on a non-empty site.");
+    if (rescale_if_off(nx, ny, nz)) rescaled++;
+    L.ni[3 * s + 0] = (float)nx;
+    L.ni[3 * s + 1] = (float)ny;
+    L.ni[3 * s + 2] = (float)nz;
   }
-  int unlisted = 0;
-  for (size_t idx = 0; idx < Nt; idx++) {
-    if (seen[idx]) continue;
-    if (L.pt[idx]) unlisted++;
-    set_empty(L, idx);
-  }
+  long unlisted = 0;
+  for (size_t s = 0; s < Nt; s++) unlisted += (L.pt[s] && !roll.named(s)) ? 1 : 0;
   if (unlisted > 0) {
-    fprintf(stderr, "  [ERROR] %s: %d non-empty grid point(s) are not listed (the grid is %d x %d x %d).\n", f,
-            unlisted, L.Nx, L.Ny, L.Nz);
+    fprintf(stderr, "  [ERROR] %s: %ld non-empty grid point(s) are not listed (the grid is %d x %d x %d).\n",
+            path.c_str(), unlisted, L.Nx, L.Ny, L.Nz);
     exit(3);
   }
-  if (normalized > 0)
-    fprintf(stderr, "  [WARNING] %d director(s) in %s did not have unit length and were normalized.\n", normalized, f);
+  if (rescaled > 0)
+    fprintf(stderr, "  [WARNING] %d director(s) in %s did not have unit length and were normalized.\n", rescaled,
+            path.c_str());
 }
 
 }  // namespace

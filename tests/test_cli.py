@@ -85,8 +85,8 @@ def close(name, value, expected, tol):
 
 # ---- input errors: the run stops, reports the problem and leaves the old output alone (run_checks.py:328-380) ----
 
-def expect_error(run, params, text, status=None, raw=False):
-    r = run(params, files={"po.dat": "old results\n"}, raw=raw)
+def expect_error(run, params, text, status=None, raw=False, files=None):
+    r = run(params, files=dict(files or {}, **{"po.dat": "old results\n"}), raw=raw)
     assert r.proc.returncode != 0, "the run did not stop"
     if status is not None:
         assert r.proc.returncode == status
@@ -153,6 +153,44 @@ def test_error_ic_file_unlisted_site(run):
             files={"start.csv": "x,y,z,nx,ny,nz\n0,0,0,0,0,1\n", "po.dat": "old results\n"})
     assert r.proc.returncode == 3 and "are not listed" in r.proc.stderr
     assert open(os.path.join(r.dir, "po.dat")).read() == "old results\n"
+
+
+# Format errors of the two column-table files (geometry_custom.cpp:62-163 with status 10, ic.cpp:103-200 with
+# status 3): each case is (file text, the message's line number, the message). The table reader reports them as
+# "  [ERROR] <file>, line <n>: <message>" and stops before any output file is touched.
+GEOM_ERRORS = [
+    ("", 1, "empty file (a header line is expected)."),
+    ("x,y,nx,ny,nz\n", 1, "missing column(s) in the header: z, pt (required: x,y,z,nx,ny,nz,pt)."),
+    ("x,y,z,nx,ny,nz,pt\n\n0,0,0,0,0,1\n", 3, "6 fields, but the header has 7."),
+    ("pt, X ,y,z,nx,ny,nz\n1,0,0,0,0,zz,1\n", 2, "column ny is not a number: 'zz'."),
+    ("x,y,z,nx,ny,nz,pt\n0,0,0,0,0,1,1.5\n", 2, "x, y, z and pt must be integers."),
+    ("x,y,z,nx,ny,nz,pt\n0,4,0,0,0,1,1\n", 2, "point (0, 4, 0) outside the 4 x 4 x 6 grid."),
+    ("x,y,z,nx,ny,nz,pt\n0,0,0,0,0,1,-1\n", 2, "negative point type (use 0 empty, 1 bulk, 2+ surface)."),
+    ("x,y,z,nx,ny,nz,pt\n1,2,3,0,0,1,1\n1,2,3,0,0,1,1\n", 3, "point (1, 2, 3) is listed twice."),
+    ("x,y,z,nx,ny,nz,pt\n0,0,0,0,0,0,2\n", 2, "surface point with a zero normal vector."),
+]
+IC_ERRORS = [
+    ("", 1, "empty file (a header line is expected)."),
+    ("x,y,z,nx,ny\n", 1, "missing column(s) in the header: nz (required: x,y,z,nx,ny,nz)."),
+    ("x,y,z,nx,ny,nz\n0,0,0,0,0\n", 2, "5 fields, but the header has 6."),
+    ("x,y,z,nx,ny,nz\n0,0,0,0,,1\n", 2, "column ny is not a number: ''."),
+    ("x,y,z,nx,ny,nz\n0,0.5,0,0,0,1\n", 2, "x, y and z must be integers."),
+    ("x,y,z,nx,ny,nz\n0,0,-1,0,0,1\n", 2, "point (0, 0, -1) outside the 4 x 4 x 6 grid."),
+    ("x,y,z,nx,ny,nz\n3,1,5,0,0,1\n\n3,1,5,0,0,1\n", 4, "point (3, 1, 5) is listed twice."),
+    ("x,y,z,nx,ny,nz\n0,0,0,0,0,0\n", 2, "zero director on a non-empty site."),
+]
+
+
+@pytest.mark.parametrize("text,line,msg", GEOM_ERRORS)
+def test_error_custom_geometry_file(run, text, line, msg):
+    params = plus(base(Nx=4, Ny=4, Nz=6, geometry="custom"), "boundary_file geom.csv")
+    expect_error(run, params, "  [ERROR] geom.csv, line %d: %s\n" % (line, msg), 10, files={"geom.csv": text})
+
+
+@pytest.mark.parametrize("text,line,msg", IC_ERRORS)
+def test_error_ic_file_format(run, text, line, msg):
+    expect_error(run, base(Nx=4, Ny=4, Nz=6, ic="ic_file", ic_file="start.csv"),
+                 "  [ERROR] start.csv, line %d: %s\n" % (line, msg), 3, files={"start.csv": text})
 
 
 # ---- known answers on the GPU (run_checks.py:113-323) ----
