@@ -46,6 +46,7 @@ _SIGS = {
     "orc_pair": (ctypes.c_float, [P, F, F, F]),
     "orc_field": (ctypes.c_float, [P, F]),
     "orc_surface": (ctypes.c_float, [P, ctypes.c_int, F, F]),
+    "orc_rp_tangent_axis": (ctypes.c_int, [F, F, F]),
     "orc_set_ghrl": (None, [P, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float, F]),
     "orc_lab_axis": (None, [ctypes.c_float, ctypes.c_float, F]),
     "orc_site_energy": (ctypes.c_float, [P, F, I, F, ctypes.c_int, F]),
@@ -162,6 +163,18 @@ def lab_axis(theta, phi):
 
 def pair(p, ni, nj, rij):
     return lib().orc_pair(ctypes.byref(p), vec(ni), vec(nj), vec(rij))
+
+
+def surface(p, surf, ni, s):
+    """Anchoring energy of surface `surf` (W included) for director ni at a site with outward normal s."""
+    return lib().orc_surface(ctypes.byref(p), int(surf), vec(ni), vec(s))
+
+
+def rp_tangent_axis(d, s):
+    """Easy axis of rp_tangent: (axis, 1) with the projection of d on the plane normal to s, or (0 vector, 0)."""
+    e = (ctypes.c_float * 3)()
+    ok = lib().orc_rp_tangent_axis(vec(d), vec(s), e)
+    return tuple(e), ok
 
 
 def geometry(name, Nx, Ny, Nz):

@@ -451,3 +451,188 @@ def test_thermal_loop_reproduces_the_c1_fixture_prefix(orc):
     f32 = lambda x: float(np.float32(x))
     rows = orc.thermal_run(p, ni, pt, None, f32(1.2), f32(1.1), f32(-0.02), 500, 200, 1, 1)
     assert np.array_equal(rows[:5], ref[:5])
+
+
+# ---- anchoring energies in closed form (anchoring_homeotropic.cpp, anchoring_rp.cpp, anchoring_rp_tangent.cpp,
+# anchoring_fg.cpp): each kind is evaluated at orientations where the value follows from the definition by hand ----
+
+def _surf_params(orc, kind, W, axis=(0.0, 0.0, 0.0), k=None):
+    kw = dict(potential="ghrl", k=k) if k is not None else {}
+    return orc.params(4, 4, 4, surfaces=[(kind, W, axis)], **kw)
+
+
+X, Y, Z = (1.0, 0.0, 0.0), (0.0, 1.0, 0.0), (0.0, 0.0, 1.0)
+R2 = 1 / math.sqrt(2)
+
+
+@pytest.mark.parametrize("W", [1.0, 2.5])
+def test_anchoring_quadratic_kinds(orc, W):
+    """homeotropic -W (n.s)^2; fg +W (n.s)^2; rp -W (n.e)^2 with e the lab axis, whatever the normal."""
+    p = _surf_params(orc, "homeotropic", W)
+    assert orc.surface(p, 0, Z, Z) == pytest.approx(-W, abs=1e-6)
+    assert orc.surface(p, 0, (0, 0, -1.0), Z) == pytest.approx(-W, abs=1e-6)
+    assert orc.surface(p, 0, X, Z) == pytest.approx(0.0, abs=1e-7)
+    assert orc.surface(p, 0, (R2, 0, R2), Z) == pytest.approx(-W / 2, abs=1e-6)
+    p = _surf_params(orc, "fg", W)
+    assert orc.surface(p, 0, Z, Z) == pytest.approx(+W, abs=1e-6)
+    assert orc.surface(p, 0, Y, Z) == pytest.approx(0.0, abs=1e-7)
+    assert orc.surface(p, 0, (0, R2, R2), Z) == pytest.approx(+W / 2, abs=1e-6)
+    axis = orc.lab_axis(90.0, 0.0)  # x
+    p = _surf_params(orc, "rp", W, axis)
+    for s in (Z, X, (0.6, 0.0, 0.8)):  # the normal plays no part in rp
+        assert orc.surface(p, 0, X, s) == pytest.approx(-W, abs=1e-6)
+        assert orc.surface(p, 0, Y, s) == pytest.approx(0.0, abs=1e-6)
+        assert orc.surface(p, 0, Z, s) == pytest.approx(0.0, abs=1e-6)
+        assert orc.surface(p, 0, (R2, R2, 0), s) == pytest.approx(-W / 2, abs=1e-6)
+
+
+def test_rp_tangent_easy_axis(orc):
+    """The easy axis is the lab axis d projected on the plane normal to s and normalized; none when d is parallel
+    (or antiparallel) to s. On the flat surface with d in-plane it is d itself."""
+    e, ok = orc.rp_tangent_axis(orc.lab_axis(45.0, 0.0), Z)
+    assert ok == 1 and e == pytest.approx(X, abs=1e-6)
+    e, ok = orc.rp_tangent_axis(orc.lab_axis(90.0, 30.0), Z)
+    assert ok == 1 and e == pytest.approx((math.cos(math.pi / 6), 0.5, 0.0), abs=1e-6)
+    e, ok = orc.rp_tangent_axis(Z, (0.6, 0.0, 0.8))  # z projected on the plane normal to (0.6, 0, 0.8)
+    assert ok == 1 and e == pytest.approx((-0.8, 0.0, 0.6), abs=1e-6)
+    for s in (Z, (0, 0, -1.0)):
+        e, ok = orc.rp_tangent_axis(Z, s)
+        assert ok == 0 and e == (0.0, 0.0, 0.0)
+    rng = np.random.default_rng(5)
+    for _ in range(50):
+        d, s = rng.normal(size=3), rng.normal(size=3)
+        d, s = d / np.linalg.norm(d), s / np.linalg.norm(s)
+        e, ok = orc.rp_tangent_axis(d, s)
+        assert ok == 1
+        e = np.array(e)
+        assert abs(np.linalg.norm(e) - 1) < 1e-6 and abs(e @ s) < 1e-6 and e @ d > 0
+        assert np.linalg.norm(np.cross(e, d - (d @ s) * s)) < 1e-5
+
+
+@pytest.mark.parametrize("W", [1.0, 3.0])
+def test_rp_tangent_energy(orc, W):
+    p = _surf_params(orc, "rp_tangent", W, orc.lab_axis(45.0, 0.0))
+    assert orc.surface(p, 0, X, Z) == pytest.approx(-W, abs=1e-6)
+    assert orc.surface(p, 0, Y, Z) == pytest.approx(0.0, abs=1e-6)
+    assert orc.surface(p, 0, Z, Z) == pytest.approx(0.0, abs=1e-6)
+    assert orc.surface(p, 0, (R2, R2, 0), Z) == pytest.approx(-W / 2, abs=1e-6)
+    # on a tilted site the axis follows the surface: d = (1, 0, 1)/sqrt 2 on s = (0.6, 0, 0.8) gives e = (0.8, 0, -0.6)
+    assert orc.surface(p, 0, (0.8, 0.0, -0.6), (0.6, 0.0, 0.8)) == pytest.approx(-W, abs=1e-5)
+    assert orc.surface(p, 0, (0.6, 0.0, 0.8), (0.6, 0.0, 0.8)) == pytest.approx(0.0, abs=1e-5)
+    assert orc.surface(p, 0, Y, (0.6, 0.0, 0.8)) == pytest.approx(0.0, abs=1e-6)
+    p = _surf_params(orc, "rp_tangent", W, Z)
+    assert orc.surface(p, 0, (-0.8, 0.0, 0.6), (0.6, 0.0, 0.8)) == pytest.approx(-W, abs=1e-6)
+    for n in (X, Y, Z, (R2, 0, R2)):  # axis parallel to the normal: no anchoring
+        assert orc.surface(p, 0, n, Z) == 0.0
+
+
+def _ghrl_k_forms(k11, k22, k33, p0=0.0):
+    """The GHRL bond U(n, m, r) at the orientations used below, written with the Frank constants only. With
+    L = 9 / |k11 - 3 k22 - k33| and l = L / 9 (io.cpp:346-357; nu = -1 when k11 < 3 k22 + k33):
+      parallel    m = n, n perpendicular to r (or m = n = r) -> -2 l (k11 - k22) - 1
+      end-on      m = r, n perpendicular to r           ->  l (1.75 k11 - 2.5 k22 + 0.75 k33) + 1/2
+      twisted     m, n perpendicular to r and each other -> l (-0.5 k11 + 2 k22 - 1.5 k33) + 1/2 - sigma
+      chiral part of U(n, m, r) - U(n', m, r) for n, n' = m rotated by +-phi about r: 2 sigma sin(phi)
+    with sigma = -L k22 2 pi / p0. The end-on form is also U(n, r, r) with n parallel to r (the bond is symmetric
+    under exchanging its ends)."""
+    L = 9.0 / abs(k11 - 3 * k22 - k33)
+    l = L / 9
+    sigma = -L * k22 * 2 * math.pi / p0 if p0 else 0.0
+    return dict(parallel=-2 * l * (k11 - k22) - 1, end_on=l * (1.75 * k11 - 2.5 * k22 + 0.75 * k33) + 0.5,
+                twisted=l * (-0.5 * k11 + 2 * k22 - 1.5 * k33) + 0.5 - sigma, sigma=sigma)
+
+
+GHRL_KS = [(1.0, 1.0, 1.0, 0.0), (1.0, 0.6, 1.4, 0.0), (1.48, 1.0, 2.28, 0.0), (1.0, 0.6, 1.4, 12.0)]
+
+
+@pytest.mark.parametrize("k", GHRL_KS)
+def test_homeotropic_ghrl_closed_form(orc, k):
+    """homeotropic_ghrl: W U(n, s, s), a phantom neighbour along the normal pointing along it."""
+    f, W = _ghrl_k_forms(*k), 1.7
+    p = _surf_params(orc, "homeotropic_ghrl", W, k=k)
+    for s in (Z, (0, 0, -1.0), (0.6, 0.0, 0.8)):
+        s = np.array(s)
+        t = np.cross(s, Y if abs(s[1]) < 0.9 else X)
+        t /= np.linalg.norm(t)
+        # n along s: the parallel form (b = 1, a_i = a_j = 1); n in the plane: the end-on form
+        assert orc.surface(p, 0, s, s) == pytest.approx(W * f["parallel"], abs=2e-5)
+        assert orc.surface(p, 0, t, s) == pytest.approx(W * f["end_on"], abs=2e-5)
+
+
+@pytest.mark.parametrize("k", GHRL_KS)
+def test_rp_ghrl_closed_form(orc, k):
+    """rp_ghrl: W U(n, e, s), a phantom neighbour along the normal s with the lab easy axis e (here x, in the
+    plane of a z surface)."""
+    f, W = _ghrl_k_forms(*k), 0.8
+    p = _surf_params(orc, "rp_ghrl", W, orc.lab_axis(90.0, 0.0), k=k)
+    assert orc.surface(p, 0, X, Z) == pytest.approx(W * f["parallel"], abs=2e-5)
+    assert orc.surface(p, 0, Z, Z) == pytest.approx(W * f["end_on"], abs=2e-5)
+    assert orc.surface(p, 0, Y, Z) == pytest.approx(W * f["twisted"], abs=2e-5)
+    phi = 0.3
+    up = orc.surface(p, 0, (math.cos(phi), math.sin(phi), 0), Z)
+    down = orc.surface(p, 0, (math.cos(phi), -math.sin(phi), 0), Z)
+    assert up - down == pytest.approx(W * 2 * f["sigma"] * math.sin(phi), abs=5e-5)
+
+
+@pytest.mark.parametrize("k", GHRL_KS)
+def test_fg_ghrl_closed_form(orc, k):
+    """fg_ghrl: W U(n, t, s) with t the normalized projection of n on the surface plane; zero when n is along the
+    normal. It is degenerate in the plane: only the tilt of n out of the plane matters."""
+    f, W = _ghrl_k_forms(*k), 1.3
+    p = _surf_params(orc, "fg_ghrl", W, k=k)
+    for n in (X, Y, (R2, -R2, 0)):
+        assert orc.surface(p, 0, n, Z) == pytest.approx(W * f["parallel"], abs=2e-5)
+    assert orc.surface(p, 0, Z, Z) == 0.0
+    assert orc.surface(p, 0, (0, 0, -1.0), Z) == 0.0
+    L = 9.0 / abs(k[0] - 3 * k[1] - k[2])
+    lam, mu, rho, nu = (L / 9 * (2 * k[0] - 3 * k[1] + k[2]), L * (k[1] - k[0]), L / 9 * (k[0] - k[2]), -1.0)
+    for theta in (0.2, 0.7, 1.2):  # n tilted by theta out of the plane: a_i = sin, a_j = 0, b = cos, c = 0
+        c, s_ = math.cos(theta), math.sin(theta)
+        P2 = 1.5 * c * c - 0.5
+        want = (1.5 * s_ * s_ - 1) * (rho * P2 + lam) - mu / 9 + nu * P2
+        for az in (0.0, 1.0, 2.5):
+            n = (c * math.cos(az), c * math.sin(az), s_)
+            assert orc.surface(p, 0, n, Z) == pytest.approx(W * want, abs=3e-5)
+
+
+# ---- Splay_magnitude (parameter_order.cpp:147-176) on constructed fields ----
+
+def _xz_rotation(Nx, Ny, Nz, q):
+    """n(x) = (sin qx, 0, cos qx): the director turns in the xz plane along x."""
+    x = np.arange(Nx)
+    n = np.zeros((Nz, Ny, Nx, 3), dtype=np.float32)
+    n[..., 0] = np.sin(q * x)
+    n[..., 2] = np.cos(q * x)
+    return n.reshape(-1)
+
+
+def test_splay_magnitude_closed_forms(orc):
+    """Only x bonds carry splay here. Summed over a ring of Nx sites the vector sum is Nt sin q cos q along z over
+    6 Nt directed bonds, so <dtheta> = 1.5 |sin q cos q| / 6 = |sin 2q| / 8. On a free x axis the end terms remain:
+    s_x = NyNz (sin^2 q(Nx-1) - 0), s_z = NyNz (sin(2q(Nx-1)) / 2 + (Nx-1) sin q cos q), over 6 Nt - 2 NyNz bonds."""
+    for Nx, m in ((12, 1), (16, 1), (16, 3), (10, 2)):
+        q = 2 * math.pi * m / Nx
+        p = orc.params(Nx, 4, 6, periodic=(1, 1, 1), potential="pear")
+        pt = np.ones(Nx * 24, dtype=np.int32)
+        got = orc.splay_magnitude(p, _xz_rotation(Nx, 4, 6, q), pt)
+        assert got == pytest.approx(abs(math.sin(2 * q)) / 8, rel=1e-5, abs=1e-7)
+    for Nx, q in ((9, 0.3), (6, 1.1)):
+        Ny, Nz = 4, 4
+        p = orc.params(Nx, Ny, Nz, periodic=(0, 1, 1), potential="pear")
+        pt = np.ones(Nx * Ny * Nz, dtype=np.int32)
+        got = orc.splay_magnitude(p, _xz_rotation(Nx, Ny, Nz, q), pt)
+        a = q * (Nx - 1)
+        sx = Ny * Nz * math.sin(a) ** 2
+        sz = Ny * Nz * (math.sin(2 * a) / 2 + (Nx - 1) * math.sin(q) * math.cos(q))
+        want = 1.5 * math.hypot(sx, sz) / (6 * Nx * Ny * Nz - 2 * Ny * Nz)
+        assert got == pytest.approx(want, rel=1e-5)
+
+
+def test_splay_magnitude_uniform_and_empty(orc):
+    for periodic in ((1, 1, 1), (0, 0, 0)):
+        p = orc.params(6, 6, 6, periodic=periodic, potential="pear")
+        pt = np.ones(216, dtype=np.int32)
+        n = np.tile(np.array([0.36, 0.48, 0.8], dtype=np.float32), 216)
+        assert orc.splay_magnitude(p, n, pt) == pytest.approx(0.0, abs=1e-7)
+    p = orc.params(6, 6, 6, potential="pear")
+    assert orc.splay_magnitude(p, np.zeros(3 * 216, dtype=np.float32), np.zeros(216, dtype=np.int32)) == 0.0
