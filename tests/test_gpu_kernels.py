@@ -9,7 +9,8 @@ import numpy as np
 import pytest
 
 from helpers import lattice_from_orc, rand_unit, sites_and_trials
-from test_oracle_known_answers import CASES, PHILOX_KAT, base_run, boltzmann_cos_hist, build_case, hedgehog
+from test_oracle_known_answers import (CASES, FLIP_CASES, PHILOX_KAT, base_run, boltzmann_cos_hist, build_case,
+                                      chi2_of, flip_case, hedgehog)
 
 pytestmark = pytest.mark.gpu
 
@@ -191,6 +192,22 @@ def test_detailed_balance_independent_spins_gpu(mcl, orc):
     tot = counts.sum()
     chi2 = (((counts - tot * prob) ** 2) / (tot * prob)).sum()
     assert chi2 < 200, chi2
+
+
+@pytest.mark.parametrize("case", FLIP_CASES, ids=lambda c: c[0])
+def test_detailed_balance_flip_trial_gpu(mcl, orc, case):
+    """The device's flip trial at T > 0 (metropolis_trial's FLIP branch, its own Philox draw per site and the fourth
+    uniform): independent spins in a polar field sample exp((elecP c + elecA c^2) / T). The pear case runs the masked
+    kernel on isolated sites."""
+    p, pt, ni, edges, prob = flip_case(orc, *case)
+    counts = np.zeros(20)
+    valid = pt == 1
+    with mcl.Engine(lattice_from_orc(mcl, p, ni, pt, None, seed=5)) as eng:
+        eng.sweeps(50)
+        for _ in range(200):
+            eng.sweeps(1)
+            counts += np.histogram(eng.directors()[2::3][valid], bins=edges)[0]
+    assert chi2_of(counts, prob) < 200, chi2_of(counts, prob)
 
 
 def test_detailed_balance_dimers_gpu(mcl, orc):

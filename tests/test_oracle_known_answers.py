@@ -406,6 +406,56 @@ def test_detailed_balance_independent_spins(orc):
     assert chi2 < 20 * 10, chi2
 
 
+def boltzmann_hist(energy, T, edges):
+    """Probabilities of cos(theta) bins for a site energy energy(cos theta) on the sphere (cos uniform a priori)."""
+    x = np.linspace(-1, 1, 200001)
+    w = np.exp(-(energy(x) - energy(x).min()) / T)
+    cdf = np.concatenate([[0], np.cumsum(0.5 * (w[1:] + w[:-1]) * np.diff(x))])
+    cdf /= cdf[-1]
+    return np.diff(np.interp(edges, x, cdf))
+
+
+# Independent spins with the flip trial on (flipMove: pear, or elecP != 0; main.cpp:88): (potential, isolated sites,
+# elecP, elecA, T). The polar field makes n and -n differ, so the n -> -n trial (evolve.cpp:197-213, evolve.cu:225-239)
+# moves weight between the hemispheres; the sampled cos(theta) must follow exp((elecP c + elecA c^2) / T). The pear
+# case empties every odd site, so no site has a neighbour (a masked lattice of isolated spins).
+FLIP_CASES = [("ll", False, 1.0, 0.0, 0.5), ("pear", True, 0.6, 0.8, 0.4)]
+
+
+def flip_case(orc, potential, isolated, elecP, elecA, T, N=16):
+    idx = np.arange(N ** 3)
+    i, j, k = idx % N, (idx // N) % N, idx // (N * N)
+    pt = ((i + j + k) % 2 == 0).astype(np.int32) if isolated else np.ones(N ** 3, dtype=np.int32)
+    p = orc.params(N, N, N, potential=potential, A=0.0 if potential == "ll" else 1.5, B1=0.09, C=0.3, elecP=elecP,
+                   elecA=elecA, elec=(0, 0, 1), T=T)
+    assert p.flipMove == 1
+    ni = orc.ic_random(pt, 11)
+    edges = np.linspace(-1, 1, 21)
+    prob = boltzmann_hist(lambda c: -elecP * c - elecA * c * c, T, edges)
+    return p, pt, ni, edges, prob
+
+
+def chi2_of(counts, prob):
+    tot = counts.sum()
+    return (((counts - tot * prob) ** 2) / (tot * prob)).sum()
+
+
+@pytest.mark.parametrize("case", FLIP_CASES, ids=lambda c: c[0])
+def test_detailed_balance_flip_trial(orc, case):
+    """The oracle's flip trial at T > 0 (chi-square over 20 bins, as above)."""
+    p, pt, ni, edges, prob = flip_case(orc, *case)
+    rngs = orc.make_rngs(2, 7)
+    av = 0.5
+    for _ in range(50):
+        _, av = orc.sweep(p, ni, pt, None, av, rngs, 2)
+    counts = np.zeros(20)
+    valid = pt == 1
+    for _ in range(200):
+        _, av = orc.sweep(p, ni, pt, None, av, rngs, 2)
+        counts += np.histogram(ni[2::3][valid], bins=edges)[0]
+    assert chi2_of(counts, prob) < 200, chi2_of(counts, prob)
+
+
 # ---- electric mode loop (evolve_electric.cpp:31-103) ----
 
 def test_electric_run_zero_field_point_equals_thermal_point(orc):
