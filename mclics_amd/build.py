@@ -42,7 +42,7 @@ def build_host():
         return
     os.makedirs(os.path.dirname(BIN), exist_ok=True)
     tmp = BIN + ".tmp"
-    _run(["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"), "-I",
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-fopenmp", "-I", os.path.join(ROOT, "include"), "-I",
           os.path.join(HERE, "csrc"), "-o", tmp] + srcs +
          ["-L", os.path.dirname(LIB), "-lmclgpu", "-Wl,-rpath,$ORIGIN/../lib", "-pthread"])
     os.replace(tmp, BIN)

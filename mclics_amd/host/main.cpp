@@ -1,9 +1,11 @@
 // mclics: Monte Carlo simulation of nematic and polar liquid crystals on a cubic lattice, with the checkerboard
-// Metropolis sweep on an MI355X through libmclgpu. Usage: mclics [input_file]. Outputs in the working directory:
-// po.dat, ic.csv and director_field_*.csv, as the reference program (src/main.cpp:32-34).
+// Metropolis sweep on an MI355X through libmclgpu (evol modes with the GPU suffix) or on the host CPU (the modes
+// without it). Usage: mclics [input_file]. Outputs in the working directory: po.dat, ic.csv and
+// director_field_*.csv, as the reference program (src/main.cpp:32-34).
 #include <dirent.h>
 #include <stdio.h>
 #include <string.h>
+#include <strings.h>
 
 #include "params.h"
 #include "run.h"
@@ -42,13 +44,16 @@ void clean_previous_output() {
 
 int main(int argc, char **argv) {
   print_header("McLics - Monte Carlo Liquid Crystal Simulator (MI355X engine)", 60, "=");
-  printf("  Sweeps and observables on the GPU through libmclgpu (HIP, gfx950).\n\n");
+  printf("  GPU modes: sweeps and observables on the MI355X through libmclgpu (HIP, gfx950).\n");
+  printf("  CPU modes: the host's OpenMP sweep.\n\n");
   Params p = read_input_file(argc > 1 ? argv[1] : nullptr);
   derive_parameters(p);
   validate_parameters(p);
   print_parameters(p);
   Lattice L = setup_simulation(p);
-  GpuRun run(p, L);
+  // the engine is set up before the previous output is removed, so a device error leaves it in place
+  const bool gpu = p.evol.size() > 3 && strcasecmp(p.evol.c_str() + p.evol.size() - 3, "GPU") == 0;
+  Run run(p, L, gpu ? make_gpu_engine(p, L) : make_cpu_engine(p, L));
   clean_previous_output();
   run.print_n((char *)"ic.csv");
   run.run();

@@ -1,10 +1,13 @@
-// GPU mode loops of the mclics host program (thermalGPU, stepGPU, quenchGPU, electricGPU) and the output
-// writers (po.dat and director_field_*.csv). The loops follow the reference's src/evolve_{thermal,step,quench,
-// electric}.cu: the same schedule, moments, column layout, number formats and file names. The sweeps, the
-// measurements (FP64 on the device) and the local order parameter of the snapshots run in libmclgpu.
+// Mode loops of the mclics host program (thermal, step, quench, electric, each with or without the GPU suffix) and
+// the output writers (po.dat and director_field_*.csv). The loops follow the reference's src/evolve_{thermal,step,
+// quench,electric}.cu and .cpp: the same schedule, moments, column layout, number formats and file names. The
+// sweeps, the FP64 measurements and the local order parameter of the snapshots come from the engine: libmclgpu for
+// the GPU modes, the program's OpenMP sweep for the CPU modes.
 #include "run.h"
 
 #include "csvfmt.h"
+
+#include <omp.h>
 
 #include <math.h>
 #include <stdio.h>
@@ -33,22 +36,6 @@ int snapshot_scale(double step) {
 }
 long snapshot_index(double x, int S) { return (long)(S * (x + 1e-3 / S)); }
 
-[[noreturn]] void gpu_fail(const char *what) {
-  fprintf(stderr, "  [ERROR] GPU (%s): %s\n", what, mcl_last_error());
-  exit(20);
-}
-
-void check(int st, const char *what) {
-  if (st != MCL_OK) gpu_fail(what);
-}
-
-// Environment switch: unset -> dflt, "0" -> false, anything else -> true.
-bool env_flag(const char *name, bool dflt) {
-  const char *v = getenv(name);
-  if (!v || !*v) return dflt;
-  return strcmp(v, "0") != 0;
-}
-
 struct Extras {
   bool pear, sph, cyl;
   void header(FILE *f) const {
@@ -74,79 +61,24 @@ struct Extras {
 
 }  // namespace
 
-GpuRun::GpuRun(Params &p, Lattice &L) : p_(p), L_(L) {
-  mcl_lattice_desc d;
-  memset(&d, 0, sizeof(d));
-  d.nx = L.Nx;
-  d.ny = L.Ny;
-  d.nz = L.Nz;
-  for (int a = 0; a < 3; a++) d.periodic[a] = L.periodic[a];
-  d.ni = L.ni.data();
-  d.pt = L.pt.data();
-  d.ns = L.ns.data();
-  d.nsurf = (int)L.surfaces.size();
-  d.surf = L.surfaces.empty() ? nullptr : L.surfaces.data();
-  mcl_model &m = d.model;
-  m.potential = iequals(p.potential, "pear") ? MCL_POT_PEAR
-                : (iequals(p.potential, "ghrl") || iequals(p.potential, "gruhn-hess")) ? MCL_POT_GHRL
-                                                                                         : MCL_POT_LL;
-  m.A = p.A;
-  m.B1 = p.B1;
-  m.C = p.C;
-  m.ghrl_lambda = p.ghrl_lambda;
-  m.ghrl_mu = p.ghrl_mu;
-  m.ghrl_nu = p.ghrl_nu;
-  m.ghrl_rho = p.ghrl_rho;
-  m.ghrl_sigma = p.ghrl_sigma;
-  m.elec_dir[0] = p.elecX;
-  m.elec_dir[1] = p.elecY;
-  m.elec_dir[2] = p.elecZ;
-  m.elecA = p.elecA;
-  m.elecP = p.elecP;
-  m.elecE = p.elecE;
-  m.flip_move = p.flipMove;
-  d.T = p.Ti;
-  d.ang_var = 0.5f;  // evolve_thermal.cu:43
-  d.seed = (uint64_t)(uint32_t)p.seed;
-  d.radial = (iequals(p.radial, "sphere") || iequals(p.radial, "both") ? MCL_RADIAL_SPHERE : 0) |
-             (iequals(p.radial, "cylinder") || iequals(p.radial, "both") ? MCL_RADIAL_CYLINDER : 0);
-  check(mcl_create(&d, 1, &ctx_), "setup");
-  // Sweeps replay captured HIP graphs of 50-sweep blocks (bit-identical to launching each kernel; MCLICS_GRAPH=0
-  // launches eagerly). MCLICS_AUTOTUNE=1 times the library's launch shapes first and keeps the fastest: the
-  // directors stay bit-identical, but the FP64 sums of the measurements follow the workgroups, so their last bits
-  // can differ between runs that pick different shapes.
-  if (env_flag("MCLICS_GRAPH", true)) check(mcl_set_graph(ctx_, 50), "setup");
-  // Small lattices run their plain sweeps resident in one workgroup's LDS (bit-identical); MCLICS_RESIDENT=0 keeps
-  // them on the two-pass kernels.
-  if (!env_flag("MCLICS_RESIDENT", true)) check(mcl_set_resident(ctx_, 0), "setup");
-  if (env_flag("MCLICS_AUTOTUNE", false)) {
-    mcl_tune_result r;
-    check(mcl_autotune(ctx_, 20, &r), "autotune");
-  }
-}
+Run::Run(Params &p, Lattice &L, std::unique_ptr<Engine> engine) : p_(p), L_(L), eng_(std::move(engine)) {}
 
-GpuRun::~GpuRun() {
-  if (ctx_) mcl_destroy(ctx_);
-}
-
-void GpuRun::sweeps(int n, int measure_every) {
+void Run::sweeps(int n, int measure_every) {
   if (n <= 0) return;
   const auto t0 = std::chrono::steady_clock::now();
-  check(mcl_sweeps(ctx_, n, measure_every), "sweep");
-  check(mcl_synchronize(ctx_), "sweep");
+  eng_->sweeps(n, measure_every);
   sweep_seconds_ += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   sweeps_ += n;
 }
 
 // Snapshot and ic.csv format (io.cpp:56-89): header x,y,z,nx,ny,nz,S,pt, x fastest, %g, S = local order
-// parameter, NaN on empty sites. The local order parameter comes from the GPU (mcl_local_order); the text is
+// parameter, NaN on empty sites. The local order parameter comes from the engine; the text is
 // formatted by up to 16 threads, each on its own chunk of lines, and the chunks are written in order, so the file is
 // byte-identical to printing the lines one by one.
-int GpuRun::print_n(const char *fname) {
+int Run::print_n(const char *fname) {
   const size_t Nt = L_.pt.size();
   std::vector<float> S(Nt);
-  check(mcl_get_directors(ctx_, 0, L_.ni.data()), "snapshot");
-  check(mcl_local_order(ctx_, 0, S.data()), "snapshot");
+  eng_->snapshot(L_.ni.data(), S.data());
   FILE *out = fopen(fname, "w");
   if (!out) {
     perror(fname);
@@ -197,7 +129,7 @@ int GpuRun::print_n(const char *fname) {
   return 0;
 }
 
-int GpuRun::run() {
+int Run::run() {
   const std::string mode = mode_base(p_.evol);
   Extras ex{iequals(p_.potential, "pear"), iequals(p_.radial, "sphere") || iequals(p_.radial, "both"),
             iequals(p_.radial, "cylinder") || iequals(p_.radial, "both")};
@@ -209,9 +141,22 @@ int GpuRun::run() {
   char fname[1000];
   mcl_moments m;
   auto measure_block = [&](void) {
-    check(mcl_reset_moments(ctx_), "moments");
+    eng_->reset_moments();
     sweeps(p_.MCS, 1);
-    check(mcl_get_moments(ctx_, 0, &m), "moments");
+    m = eng_->moments();
+  };
+  // the CPU modes name their thread count after the po.dat header (evolve_thermal.cpp:56-58)
+  const bool cpu = strcmp(eng_->device(), "CPU") == 0;
+  auto threads_line = [&](void) {
+    if (!cpu) return;
+    print_field("Threads:", omp_get_max_threads());
+    printf("\n");
+    fflush(stdout);
+  };
+  char label[32];
+  auto title = [&](const char *name) {
+    snprintf(label, sizeof(label), cpu ? "  %s evolution\n" : "  %s evolution (GPU)\n", name);
+    printf("%s", label);
   };
   auto write_line = [&](const char *first) {
     const double S1 = m.S / m.n, S2 = m.S2 / m.n, E = m.E / m.n, E2 = m.E2 / m.n;
@@ -221,7 +166,7 @@ int GpuRun::run() {
   };
   char first[64];
   if (iequals(mode, "thermal")) {
-    printf("  Thermal evolution (GPU)\n");
+    title("Thermal");
     print_field("Ti:", p_.Ti);
     print_field("Tf:", p_.Tf);
     print_field("dT:", p_.dT);
@@ -229,12 +174,13 @@ int GpuRun::run() {
     fprintf(po, "T S varS E varE");
     ex.header(po);
     fflush(po);
+    threads_line();
     const int snapS = snapshot_scale(p_.dT);
     if (snapS != 100) printf("  Snapshot names: director_field_<int(%d T)>.csv (|dT| < 0.01)\n", snapS);
     const int nT = sweep_points(p_.Ti, p_.Tf, p_.dT);
     for (int it = 0; it < nT; it++) {
       p_.T = (float)sweep_value(p_.Ti, p_.Tf, p_.dT, it, nT);
-      check(mcl_set_temperature(ctx_, 0, p_.T), "temperature");
+      eng_->set_temperature(p_.T);
       sweeps(p_.MCT, 0);
       measure_block();
       snprintf(fname, sizeof(fname), "director_field_%ld.csv", snapshot_index(p_.T, snapS));
@@ -243,15 +189,16 @@ int GpuRun::run() {
       write_line(first);
     }
   } else if (iequals(mode, "step")) {
-    printf("  Step evolution (GPU)\n");
+    title("Step");
     print_field("First file:", p_.first_file);
     print_field("Last file:", p_.first_file + p_.fn);
     printf("\n");
     fprintf(po, "ii S varS E varE");
     ex.header(po);
     fflush(po);
+    threads_line();
     p_.T = p_.Ti;
-    check(mcl_set_temperature(ctx_, 0, p_.T), "temperature");
+    eng_->set_temperature(p_.T);
     for (int ii = p_.first_file; ii < p_.fn + p_.first_file; ii++) {
       sweeps(p_.MCT, 0);
       measure_block();
@@ -261,7 +208,7 @@ int GpuRun::run() {
       write_line(first);
     }
   } else if (iequals(mode, "quench")) {
-    printf("  Quench evolution (GPU)\n");
+    title("Quench");
     printf("  %-13s%g (MCT = %d steps)\n", "Ti:", p_.Ti, p_.MCT);
     printf("  %-13s%g (MCQ = %d steps, then MCS = %d measurement steps)\n", "Tf:", p_.Tf, p_.MCQ, p_.MCS);
     printf("  %-13s%d\n", "Repetitions:", p_.fn);
@@ -269,12 +216,13 @@ int GpuRun::run() {
     fprintf(po, "ii S varS E varE");
     ex.header(po);
     fflush(po);
+    threads_line();
     for (int ii = p_.first_file; ii < p_.fn + p_.first_file; ii++) {
       p_.T = p_.Ti;
-      check(mcl_set_temperature(ctx_, 0, p_.T), "temperature");
+      eng_->set_temperature(p_.T);
       sweeps(p_.MCT, 0);
       p_.T = p_.Tf;
-      check(mcl_set_temperature(ctx_, 0, p_.T), "temperature");
+      eng_->set_temperature(p_.T);
       sweeps(p_.MCQ, 0);
       measure_block();
       snprintf(fname, sizeof(fname), "director_field_%d.csv", ii);
@@ -283,22 +231,23 @@ int GpuRun::run() {
       write_line(first);
     }
   } else {  // electric
-    printf("  Electric evolution (GPU)\n");
+    title("Electric");
     print_field("Ei:", p_.elecEi);
     print_field("Ef:", p_.elecEf);
     print_field("dE:", p_.elecdE);
     printf("\n");
     p_.T = p_.Ti;
-    check(mcl_set_temperature(ctx_, 0, p_.T), "temperature");
+    eng_->set_temperature(p_.T);
     fprintf(po, "field S varS E varE");
     ex.header(po);
     fflush(po);
+    threads_line();
     const int snapS = snapshot_scale(p_.elecdE);
     printf("  Snapshot names: director_field_<int(%d E)>.csv\n", snapS);
     const int nE = sweep_points(p_.elecEi, p_.elecEf, p_.elecdE);
     for (int ie = 0; ie < nE; ie++) {
       p_.elecE = (float)sweep_value(p_.elecEi, p_.elecEf, p_.elecdE, ie, nE);
-      check(mcl_set_field(ctx_, 0, p_.elecE), "field");
+      eng_->set_field(p_.elecE);
       sweeps(p_.MCT, 0);
       measure_block();
       snprintf(fname, sizeof(fname), "director_field_%ld.csv", snapshot_index(p_.elecE, snapS));
@@ -309,7 +258,7 @@ int GpuRun::run() {
   }
   fclose(po);
   const double updates = (double)sweeps_ * (double)L_.nvalid;
-  printf("\n  GPU sweeps: %lld, site updates: %.6g, sweep time: %.3f s (%.3f site-updates/ns)\n\n",
+  printf("\n  %s sweeps: %lld, site updates: %.6g, sweep time: %.3f s (%.3f site-updates/ns)\n\n", eng_->device(),
          (long long)sweeps_, updates, sweep_seconds_,
          sweep_seconds_ > 0 ? updates / (sweep_seconds_ * 1e9) : 0.0);
   return 0;

@@ -358,9 +358,8 @@ void ic_file(Lattice &L, const std::string &path) {
 }  // namespace
 
 Lattice setup_simulation(Params &p) {
-  // evolution mode (simulator.cpp:162-188): this program runs the GPU modes
+  // evolution mode (simulator.cpp:36-53): thermal, step, quench or electric, on the CPU or with the GPU suffix
   const std::string mode = mode_base(p.evol);
-  const bool gpu = p.evol.size() > 3 && strcasecmp(p.evol.c_str() + p.evol.size() - 3, "GPU") == 0;
   const bool known = iequals(mode, "thermal") || iequals(mode, "step") || iequals(mode, "quench") ||
                      iequals(mode, "electric");
   if (!known) {
@@ -368,12 +367,6 @@ Lattice setup_simulation(Params &p) {
             "  [ERROR] Evol mode '%s' not implemented. Options: thermal | step | quench | electric, or the same"
             " with the GPU suffix (thermalGPU, ...)\n",
             p.evol.c_str());
-    exit(2);
-  }
-  if (!gpu) {
-    fprintf(stderr,
-            "  [ERROR] Evol mode '%s' runs on the CPU, which this MI355X build does not provide; use %sGPU.\n",
-            p.evol.c_str(), mode.c_str());
     exit(2);
   }
   Lattice L;

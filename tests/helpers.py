@@ -30,3 +30,13 @@ def sites_and_trials(pt, n, seed):
     valid = np.flatnonzero(pt)
     sites = rng.choice(valid, size=min(n, len(valid)), replace=False).astype(np.int32)
     return sites, rand_unit(rng, len(sites))
+
+
+def z_scores(a, b):
+    """a, b: (replicas, points) samples of one quantity from two samplers -> z per point:
+    (mean_a - mean_b) / sqrt(s_a^2 / n_a + s_b^2 / n_b), s the replica-to-replica standard deviation."""
+    n_a, n_b = a.shape[0], b.shape[0]
+    d = a.mean(0) - b.mean(0)
+    s = np.sqrt(a.var(0, ddof=1) / n_a + b.var(0, ddof=1) / n_b)
+    s = np.where(s > 0, s, 1e-300)
+    return d / s

@@ -1,9 +1,10 @@
-"""The mclics program as a drop-in for the reference's GPU modes: the known-answer checks of the reference's
-tests/run_checks.py (run with its --gpu switch, i.e. evol <mode>GPU), re-expressed as pytest.
+"""The mclics program as a drop-in for the reference's modes: the known-answer checks of the reference's
+tests/run_checks.py, re-expressed as pytest, in the GPU modes (evol <mode>GPU, run_checks.py --gpu) and in the CPU
+modes (evol <mode>, the program's OpenMP sweep).
 
-The input-error checks stop before any device call and run without a GPU; every other check runs the program on
-the GPU. Each check writes an input file into a fresh folder, runs the program there and reads po.dat and the
-snapshots.
+The input-error checks stop before any device call and run without a GPU; the known-answer checks run once per
+mode family (the GPU variants are marked gpu). Each check writes an input file into a fresh folder, runs the program
+there and reads po.dat and the snapshots.
 """
 import math
 import os
@@ -42,14 +43,14 @@ def run(tmp_path):
     assert os.access(PROGRAM, os.X_OK), "mclics is not built (python -m mclics_amd.build)"
     count = [0]
 
-    def go(params, files=None, setup=None, raw=False, env=None):
+    def go(params, files=None, setup=None, raw=False, env=None, suffix="GPU"):
         count[0] += 1
         d = tmp_path / ("run%03d" % count[0])
         d.mkdir()
         lines = []
         for k, v in (params.items() if isinstance(params, dict) else params):
             if k.lower() in ("evol", "mode") and not raw and not str(v).lower().endswith("gpu"):
-                v = str(v) + "GPU"
+                v = str(v) + suffix
             lines.append("%s %s" % (k, v))
         (d / "input.txt").write_text("\n".join(lines) + "\n")
         for name, text in (files or {}).items():
@@ -62,6 +63,14 @@ def run(tmp_path):
                            env=dict(os.environ, **(env or {})))
         return Run(str(d), p)
 
+    return go
+
+
+@pytest.fixture(params=[pytest.param("GPU", marks=pytest.mark.gpu, id="gpu"), pytest.param("", id="cpu")])
+def krun(request, run):
+    """run() in the GPU modes and, as a second case, in the CPU modes."""
+    def go(params, **kw):
+        return run(params, suffix=request.param, **kw)
     return go
 
 
@@ -133,11 +142,6 @@ def test_error_radial(run):
     expect_error(run, base(radial="spiral"), "radial must be sphere, cylinder, both or none", 1)
 
 
-def test_error_cpu_mode_rejected(run):
-    """This build has no CPU sweep: a CPU mode name stops with a message and changes nothing."""
-    expect_error(run, base(evol="thermal"), "use thermalGPU", 2, raw=True)
-
-
 def test_error_unknown_mode(run):
     expect_error(run, base(evol="anneal"), "Evol mode 'annealGPU' not implemented", 2)
 
@@ -193,43 +197,38 @@ def test_error_ic_file_format(run, text, line, msg):
                  "  [ERROR] start.csv, line %d: %s\n" % (line, msg), 3, files={"start.csv": text})
 
 
-# ---- known answers on the GPU (run_checks.py:113-323) ----
+# ---- known answers in the GPU and CPU modes (run_checks.py:113-323) ----
 
 pytestmark_gpu = pytest.mark.gpu
 
 
-@pytestmark_gpu
-def test_ground_state_ll(run):
-    r = run(base()).ok()
+def test_ground_state_ll(krun):
+    r = krun(base()).ok()
     _, rows = r.po()
     close("E", rows[0]["E"], -9.0, 1e-4)
     close("S", rows[0]["S"], 1.0, 1e-5)
     close("varE", rows[0]["varE"], 0.0, 1e-8)
 
 
-@pytestmark_gpu
-def test_ground_state_ghrl_equal_k(run):
-    _, rows = run(base(potential="ghrl", k11=1, k22=1, k33=1)).ok().po()
+def test_ground_state_ghrl_equal_k(krun):
+    _, rows = krun(base(potential="ghrl", k11=1, k22=1, k33=1)).ok().po()
     close("E", rows[0]["E"], -6.0, 1e-4)
 
 
-@pytestmark_gpu
-def test_ground_state_pear(run):
-    _, rows = run(base(potential="pear", A=1.5, B1=0.09, C=0.3)).ok().po()
+def test_ground_state_pear(krun):
+    _, rows = krun(base(potential="pear", A=1.5, B1=0.09, C=0.3)).ok().po()
     close("E", rows[0]["E"], 6 * (-1.5 - 0.09), 1e-4)
     close("P", rows[0]["P"], 1.0, 1e-5)
 
 
-@pytestmark_gpu
-def test_ground_state_slab_homeotropic(run):
-    _, rows = run(plus(base(Nz=6, geometry="slab"), "anchoring_type 0 homeotropic", "W 0 1",
+def test_ground_state_slab_homeotropic(krun):
+    _, rows = krun(plus(base(Nz=6, geometry="slab"), "anchoring_type 0 homeotropic", "W 0 1",
                        "anchoring_type 1 homeotropic", "W 1 1")).ok().po()
     close("E", rows[0]["E"], (-1.5 * (256 * 6 + 128 * 5) - 128) / 384, 1e-4)
 
 
-@pytestmark_gpu
-def test_ground_state_field(run):
-    _, rows = run(base(elecA=0.5, elecX=0, elecY=0, elecZ=2)).ok().po()
+def test_ground_state_field(krun):
+    _, rows = krun(base(elecA=0.5, elecX=0, elecY=0, elecZ=2)).ok().po()
     close("E", rows[0]["E"], -9.5, 1e-4)
 
 
@@ -246,11 +245,10 @@ def hedgehog_csv(n, sign=1):
     return "\n".join(out) + "\n"
 
 
-@pytestmark_gpu
-def test_hedgehog_radial_order(run):
+def test_hedgehog_radial_order(krun):
     p = plus(base(Nx=21, Ny=21, Nz=21, geometry="sphere", potential="pear", A=1.5, B1=0.09, C=0.3, ic="ic_file",
                   ic_file="hedgehog.csv"), "anchoring_type 0 homeotropic", "W 0 1.5")
-    header, rows = run(p, files={"hedgehog.csv": hedgehog_csv(21)}).ok().po()
+    header, rows = krun(p, files={"hedgehog.csv": hedgehog_csv(21)}).ok().po()
     for col in ("P", "dtheta", "divn", "Sr_sph", "Pr_sph"):
         assert col in header
     close("Sr_sph", rows[0]["Sr_sph"], 1.0, 0.03)
@@ -258,67 +256,59 @@ def test_hedgehog_radial_order(run):
     close("divn", rows[0]["divn"], 0.275, 0.02)
 
 
-@pytestmark_gpu
-def test_columns_ll_sphere(run):
-    header, _ = run(plus(base(Nx=11, Ny=11, Nz=11, geometry="sphere", ic="random"), "anchoring_type 0 homeotropic",
+def test_columns_ll_sphere(krun):
+    header, _ = krun(plus(base(Nx=11, Ny=11, Nz=11, geometry="sphere", ic="random"), "anchoring_type 0 homeotropic",
                          "W 0 1")).ok().po()
     assert header == ["T", "S", "varS", "E", "varE", "Sr_sph"]
 
 
-@pytestmark_gpu
-def test_thermal_sweep_points(run):
-    r = run(base(Nx=4, Ny=4, Nz=4, Ti=1.0, Tf=0.5, dT=-0.1, ic="random")).ok()
+def test_thermal_sweep_points(krun):
+    r = krun(base(Nx=4, Ny=4, Nz=4, Ti=1.0, Tf=0.5, dT=-0.1, ic="random")).ok()
     _, rows = r.po()
     T = [row["T"] for row in rows]
     assert len(T) == 6 and all(abs(a - b) <= 1e-6 for a, b in zip(T, [1.0, 0.9, 0.8, 0.7, 0.6, 0.5]))
     assert r.snapshots() == sorted("director_field_%d.csv" % v for v in (100, 90, 80, 70, 60, 50))
 
 
-@pytestmark_gpu
-def test_thermal_fine_step_names(run):
-    r = run(base(Nx=4, Ny=4, Nz=4, Ti=1.24, Tf=1.224, dT=-0.004, ic="random")).ok()
+def test_thermal_fine_step_names(krun):
+    r = krun(base(Nx=4, Ny=4, Nz=4, Ti=1.24, Tf=1.224, dT=-0.004, ic="random")).ok()
     _, rows = r.po()
     assert len(rows) == 5
     assert r.snapshots() == sorted("director_field_%d.csv" % v for v in (1240, 1236, 1232, 1228, 1224))
 
 
-@pytestmark_gpu
-def test_thermal_step_001_names(run):
-    r = run(base(Nx=4, Ny=4, Nz=4, Ti=1.2, Tf=1.1, dT=-0.01, ic="random")).ok()
+def test_thermal_step_001_names(krun):
+    r = krun(base(Nx=4, Ny=4, Nz=4, Ti=1.2, Tf=1.1, dT=-0.01, ic="random")).ok()
     assert r.snapshots() == sorted("director_field_%d.csv" % v for v in range(110, 121))
 
 
-@pytestmark_gpu
-def test_electric_sweep(run):
-    header, rows = run(base(Nx=4, Ny=4, Nz=4, evol="electric", Ti=0.5, elecA=0.1, elecZ=1, Ei=0, Ef=1.2, dE=0.1,
+def test_electric_sweep(krun):
+    header, rows = krun(base(Nx=4, Ny=4, Nz=4, evol="electric", Ti=0.5, elecA=0.1, elecZ=1, Ei=0, Ef=1.2, dE=0.1,
                             ic="random")).ok().po()
     assert header[0] == "field"
     E = [row["field"] for row in rows]
     assert len(E) == 13 and abs(E[-1] - 1.2) <= 1e-6
 
 
-@pytestmark_gpu
-def test_step_blocks(run):
-    r = run(base(Nx=4, Ny=4, Nz=4, evol="step", Ti=0.5, fn=3, first_file_number=5, MCT=2, ic="random")).ok()
+def test_step_blocks(krun):
+    r = krun(base(Nx=4, Ny=4, Nz=4, evol="step", Ti=0.5, fn=3, first_file_number=5, MCT=2, ic="random")).ok()
     header, rows = r.po()
     assert header[0] == "ii" and [row["ii"] for row in rows] == [5, 6, 7]
     assert r.snapshots() == ["director_field_5.csv", "director_field_6.csv", "director_field_7.csv"]
 
 
-@pytestmark_gpu
-def test_quench_relaxes(run):
-    _, rows = run(base(evol="quench", Ti=2.0, Tf=0.5, MCT=50, MCQ=300, MCS=20, fn=1, ic="random")).ok().po()
+def test_quench_relaxes(krun):
+    _, rows = krun(base(evol="quench", Ti=2.0, Tf=0.5, MCT=50, MCQ=300, MCS=20, fn=1, ic="random")).ok().po()
     assert rows[0]["S"] > 0.8
 
 
-@pytestmark_gpu
-def test_snapshot_as_ic_file(run):
-    r1 = run(base(Nx=6, Ny=6, Nz=6, Ti=0.3, Tf=0.3, MCT=5, ic="random")).ok()
+def test_snapshot_as_ic_file(krun):
+    r1 = krun(base(Nx=6, Ny=6, Nz=6, Ti=0.3, Tf=0.3, MCT=5, ic="random")).ok()
     snap = os.path.join(r1.dir, "director_field_30.csv")
 
     def copy(d):
         shutil.copy(snap, os.path.join(d, "start.csv"))
-    r2 = run(base(Nx=6, Ny=6, Nz=6, ic="ic_file", ic_file="start.csv"), setup=copy).ok()
+    r2 = krun(base(Nx=6, Ny=6, Nz=6, ic="ic_file", ic_file="start.csv"), setup=copy).ok()
 
     def directors(path):
         with open(path) as f:
@@ -327,21 +317,19 @@ def test_snapshot_as_ic_file(run):
     assert directors(snap) == directors(os.path.join(r2.dir, "ic.csv"))
 
 
-@pytestmark_gpu
-def test_same_seed_same_run(run):
+def test_same_seed_same_run(krun):
     p = base(Nx=6, Ny=6, Nz=6, Ti=0.8, Tf=0.8, MCT=20, MCS=20, ic="random")
-    a, b = run(p).ok(), run(p).ok()
+    a, b = krun(p).ok(), krun(p).ok()
     p["seed"] = 2
-    c = run(p).ok()
+    c = krun(p).ok()
     po = [open(os.path.join(x.dir, "po.dat")).read() for x in (a, b, c)]
     assert po[0] == po[1]
     assert po[0] != po[2]
 
 
-@pytestmark_gpu
-def test_snapshot_format(run):
+def test_snapshot_format(krun):
     """x,y,z,nx,ny,nz,S,pt with x fastest, NaN for S on empty sites (io.cpp:56-89)."""
-    r = run(plus(base(Nx=7, Ny=7, Nz=7, geometry="sphere", ic="random", MCS=2), "anchoring_type 0 homeotropic",
+    r = krun(plus(base(Nx=7, Ny=7, Nz=7, geometry="sphere", ic="random", MCS=2), "anchoring_type 0 homeotropic",
                  "W 0 1")).ok()
     lines = open(os.path.join(r.dir, "ic.csv")).read().splitlines()
     assert lines[0] == "x,y,z,nx,ny,nz,S,pt" and len(lines) == 1 + 7 ** 3
@@ -358,14 +346,13 @@ def read_ic(r, name="ic.csv"):
         return [l.strip().split(",") for l in f if l.strip()]
 
 
-@pytestmark_gpu
-def test_cholesteric_ic(run):
+def test_cholesteric_ic(krun):
     """ic cholesteric (ic.cpp:202-225): on every non-empty site phi = phi_0 - 2 pi k / p0 and n = (sin theta_0 cos phi,
     sin theta_0 sin phi, cos theta_0), with the reference's float / double steps; p0_i replaces p0 when set."""
     import numpy as np
     f32 = np.float32
     for extra, pitch in (((), 8.0), (("p0_i 5",), 5.0)):
-        r = run(plus(base(Nx=4, Ny=2, Nz=12, ic="cholesteric", theta_0=70, phi_0=30), "p0 8", *extra)).ok()
+        r = krun(plus(base(Nx=4, Ny=2, Nz=12, ic="cholesteric", theta_0=70, phi_0=30), "p0 8", *extra)).ok()
         rows = read_ic(r)
         assert len(rows) == 4 * 2 * 12
         phi_0 = f32(30 * math.pi / 180)
@@ -378,8 +365,7 @@ def test_cholesteric_ic(run):
                 close("n%d at k=%d" % (a, k), float(row[3 + a]), float(want[a]), 2e-6)
 
 
-@pytestmark_gpu
-def test_custom_geometry_ground_state(run):
+def test_custom_geometry_ground_state(krun):
     """geometry custom with a boundary file (geometry_custom.cpp:70-163): plane k = 0 empty, plane k = 3 a homeotropic
     surface with normal z (pt 2), the rest unlisted and therefore bulk. Periodic 4 x 4 x 6, uniform along z at T = 0:
     the sites next to the empty plane have 5 bonds (-7.5), the surface plane adds -W, so
@@ -389,7 +375,7 @@ def test_custom_geometry_ground_state(run):
         for i in range(4):
             lines.append("%d,%d,0,0,0,1,0" % (i, j))
             lines.append("%d,%d,3,0,0,1,2" % (i, j))
-    r = run(plus(base(Nx=4, Ny=4, Nz=6, geometry="custom"), "boundary_file geom.csv", "anchoring_type 0 homeotropic",
+    r = krun(plus(base(Nx=4, Ny=4, Nz=6, geometry="custom"), "boundary_file geom.csv", "anchoring_type 0 homeotropic",
                  "W 0 2"), files={"geom.csv": "\n".join(lines) + "\n"}).ok()
     _, rows = r.po()
     close("E", rows[0]["E"], (2 * -7.5 + 2 * -9 + (-9 - 2)) / 5, 1e-4)
@@ -414,4 +400,28 @@ def test_launch_switches_leave_outputs_identical(run):
             files = ["po.dat"] + r.snapshots()
             outs.append({f: open(os.path.join(r.dir, f)).read() for f in files})
         assert len(outs[0]) >= 3
+        assert outs[0] == outs[1] == outs[2]
+
+
+def test_cpu_modes_do_not_depend_on_the_thread_count(run):
+    """The CPU engine keys every trial's random numbers by (seed, site, sweep) and sums its measurements per plane in
+    plane order, so po.dat and every snapshot are byte-identical for any number of OpenMP threads (the reference's
+    per-thread generators make its CPU trajectories change with OMP_NUM_THREADS). Cases: a GHRL slab with planar
+    and homeotropic anchoring and a field in the electric mode; a pear sphere (flip trial, radial and pear columns)
+    in the thermal mode."""
+    cases = [plus(base(Nx=8, Ny=6, Nz=10, geometry="slab", zbound="free", ic="random", potential="ghrl", evol="electric",
+                       Ti=0.5, elecA=0.3, elecX=1, Ei=0, Ef=0.4, dE=0.2, MCT=20, MCS=10), "k11 1.0", "k22 0.6",
+                  "k33 1.4", "anchoring_type 0 rp", "W 0 1.5", "theta_s 0 90", "phi_s 0 30",
+                  "anchoring_type 1 homeotropic", "W 1 1"),
+             plus(base(Nx=9, Ny=9, Nz=9, geometry="sphere", ic="random", potential="pear", A=1.5, B1=0.09, C=0.3,
+                       Ti=0.6, Tf=0.5, dT=-0.1, MCT=20, MCS=10, radial="both"), "anchoring_type 0 homeotropic",
+                  "W 0 1")]
+    for params in cases:
+        outs = []
+        for threads in ("1", "3", "8"):
+            r = run(params, suffix="", env={"OMP_NUM_THREADS": threads}).ok()
+            assert ("Threads:     %s" % threads) in r.proc.stdout
+            files = ["po.dat", "ic.csv"] + r.snapshots()
+            outs.append({f: open(os.path.join(r.dir, f)).read() for f in files})
+        assert len(outs[0]) >= 4
         assert outs[0] == outs[1] == outs[2]

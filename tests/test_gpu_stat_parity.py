@@ -18,6 +18,8 @@ import os
 import numpy as np
 import pytest
 
+from helpers import z_scores
+
 HERE = os.path.dirname(os.path.abspath(__file__))
 FIXTURE = os.path.join(HERE, "golden", "c1_oracle.npz")
 
@@ -45,15 +47,6 @@ def run_gpu_c1(mcl, orc, seeds):
                 S1, S2, E1, E2 = m["S"] / m["n"], m["S2"] / m["n"], m["E"] / m["n"], m["E2"] / m["n"]
                 rows[r, it] = (T, S1, S2 - S1 * S1, E1, E2 - E1 * E1)
     return rows
-
-
-def z_scores(gpu, cpu):
-    """gpu, cpu: (replicas, points) arrays -> z per point."""
-    n_g, n_c = gpu.shape[0], cpu.shape[0]
-    d = gpu.mean(0) - cpu.mean(0)
-    s = np.sqrt(gpu.var(0, ddof=1) / n_g + cpu.var(0, ddof=1) / n_c)
-    s = np.where(s > 0, s, 1e-300)
-    return d / s
 
 
 def test_c1_thermal_parity_64_replicas(mcl, orc):
