@@ -6,8 +6,10 @@ homogeneous along z, fixed T = 1.10 (just below T_NI ~ 1.123), Metropolis rotati
 maximum angle (ang_var rule, about 50 % acceptance), FP64 lattice reductions for S and E every 10 sweeps.
 A step is one sweep: one Metropolis trial per site of the lattice. The default run times 20000 sweeps, the
 length of the benchmark configuration, after 2000 warmup sweeps; the directors stay resident in HBM throughout.
-The sweeps run as replays of captured HIP graphs of 50 sweeps (the library's production path), with the launch shape
-from the library's autotuner. The measurement every 10 sweeps is folded into the sweep kernel itself.
+The sweeps run as replays of captured HIP graphs (the library's production path: 50-sweep blocks, and one graph of
+the remaining length for a call that is not a multiple of 50, such as a 20-sweep run), with the launch shape from the
+library's autotuner; the graphs are captured before the warmup. The measurement every 10 sweeps is folded into the
+sweep kernel itself.
 
     python bench.py [--gpus 1] [--steps 20000] [--warmup 2000] [--no-cpu-baseline]
 
@@ -405,6 +407,9 @@ def main():
         log("bench: launch shape %s" % tune)
         if a.graph:
             eng.set_graph(a.graph)  # replay captured graphs; the events then bracket each replay
+            # capture the timed call's graphs now (nothing runs), before the warmup, so that the warmup sweeps run
+            # right up to the timed region
+            eng.prepare(a.steps, MEASURE_EVERY)
         eng.sweeps(a.warmup, MEASURE_EVERY)
         eng.synchronize()
         eng.reset_moments()

@@ -197,10 +197,16 @@ int mcl_autotune(void *ctx, int32_t trial_sweeps, mcl_tune_result *out);
 int mcl_autotune_ex(void *ctx, int32_t trial_sweeps, int32_t measure_every, mcl_tune_result *out);
 /* Sets the launch shape by hand (threads a multiple of 64, at most 1024). */
 int mcl_set_launch(void *ctx, int32_t threads, int32_t xcd_order);
-/* block_sweeps > 0: mcl_sweeps replays captured HIP graphs of about block_sweeps sweeps (a multiple of
- * measure_every) instead of launching each kernel; the rest of a call runs eagerly. 0 switches graphs off.
- * Results are identical to the eager launches. */
+/* block_sweeps > 0: mcl_sweeps replays captured HIP graphs instead of launching each kernel: graphs of about
+ * block_sweeps sweeps (a multiple of measure_every) while the call has that many left, then one graph of the largest
+ * multiple of measure_every that is left (the whole rest when nothing is measured), so calls of any length that is a
+ * multiple of measure_every run from graphs; fewer than measure_every remaining sweeps run eagerly. Each context
+ * keeps the 8 graphs it replayed last. 0 switches graphs off. Results are identical to the eager launches. */
 int mcl_set_graph(void *ctx, int32_t block_sweeps);
+/* Captures (without running anything) the graphs that mcl_sweeps(ctx, nsweeps, measure_every) would replay under
+ * the current launch shape and graph setting, so that the capture does not fall inside a timed or latency-bound
+ * call. No reference counterpart. Does nothing when graphs are off or the call would run resident sweeps. */
+int mcl_prepare_sweeps(void *ctx, int32_t nsweeps, int32_t measure_every);
 /* Resident sweeps (no reference counterpart; for the small lattices of campaigns such as the reference's paper runs,
  * whose sweeps are otherwise bound by the latency of two launches each). When every lattice of the context has at
  * most 4096 colour-block positions, 2 ceil(nx/2) ny nz (2048 with the flip trial: 16^3, 10 x 10 x 16 and 6 x 6 x 48

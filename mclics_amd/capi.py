@@ -99,6 +99,7 @@ ABI = {
     "mcl_autotune_ex": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(TuneResult)]),
     "mcl_set_launch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]),
     "mcl_set_graph": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
+    "mcl_prepare_sweeps": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]),
     "mcl_set_resident": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "mcl_resident": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32)]),
     "mcl_mlp_create": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_uint64,
@@ -318,6 +319,10 @@ class Engine:
 
     def set_graph(self, block_sweeps):
         _check(self.lib.mcl_set_graph(self.h, int(block_sweeps)))
+
+    def prepare(self, n, measure_every=0):
+        """mcl_prepare_sweeps: captures the graphs of sweeps(n, measure_every) without running them."""
+        _check(self.lib.mcl_prepare_sweeps(self.h, int(n), int(measure_every)))
 
     def set_resident(self, enable):
         """mcl_set_resident: plain sweeps of small lattices in one launch per block (bit-identical trajectories)."""

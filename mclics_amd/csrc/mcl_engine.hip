@@ -596,27 +596,32 @@ constexpr int kNQS = 8;
 // radial and pear observables are zero in this layout.
 __device__ __forceinline__ void finalize_compact(int r, const double *__restrict__ part, int stride, int2 rg,
                                  MomDev *__restrict__ mom, ObsDev *__restrict__ obs) {
-  __shared__ double fred[16][kNQS];
-  double v[kNQS];
+  // The summation order of k_finalize whatever the sweep's block size: kFinWaves virtual waves of 64 lanes, lane l of
+  // virtual wave vw adding partials rg.x + 64 vw + l, + kFinBlock, ...; each virtual wave summed by wave_sum; the
+  // wave sums added in order. A measured sweep then gets the same moments, bit for bit, whether its finalize runs
+  // here (in the next colour-0 pass) or in k_finalize (after the last sweep of a call or of a captured graph).
+  __shared__ double fred[kFinWaves][kNQS];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = (int)(blockDim.x >> 6);
+  for (int vw = w; vw < kFinWaves; vw += nw) {
+    double v[kNQS];
 #pragma unroll
-  for (int a = 0; a < kNQS; a++) v[a] = 0.0;
-  for (int p = rg.x + (int)threadIdx.x; p < rg.y; p += (int)blockDim.x) {
+    for (int a = 0; a < kNQS; a++) v[a] = 0.0;
+    for (int p = rg.x + 64 * vw + lane; p < rg.y; p += kFinBlock) {
 #pragma unroll
-    for (int a = 0; a < kNQS; a++) v[a] += part[(size_t)a * stride + p];
-  }
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+      for (int a = 0; a < kNQS; a++) v[a] += part[(size_t)a * stride + p];
+    }
 #pragma unroll
-  for (int a = 0; a < kNQS; a++) {
-    const double x = wave_sum(v[a]);
-    if (lane == 0) fred[w][a] = x;
+    for (int a = 0; a < kNQS; a++) {
+      const double x = wave_sum(v[a]);
+      if (lane == 0) fred[vw][a] = x;
+    }
   }
   __syncthreads();
   if (threadIdx.x != 0) return;
   double t[kNQS];
-  const int nw = (int)(blockDim.x >> 6);
   for (int a = 0; a < kNQS; a++) {
     double x = 0.0;
-    for (int ww = 0; ww < nw; ww++) x += fred[ww][a];
+    for (int ww = 0; ww < kFinWaves; ww++) x += fred[ww][a];
     t[a] = x;
   }
   const double N = t[6], inv = N > 0.0 ? 1.0 / N : 0.0;
@@ -884,6 +889,8 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(MEAS && (M
     static_assert(kNQS == 8, "row_rs8 reduces 8 quantities");
     // per wave and row of 16 lanes, the row sums of the 8 quantities (lanes 0-7 of each row write them). Only in
     // the instances that run at 6 waves per SIMD: under the 8-wave bound of the others the reduce-scatter spills.
+    // The measuring passes that get here are those of GHRL contexts and of masked LL contexts without empty sites
+    // (slabs): pear contexts and contexts with empty sites measure with the separate kernels.
     __shared__ double mred[16][4][kNQS];
     const double x = row_rs8(qm, lane);
     if ((lane & 8) == 0) mred[threadIdx.x >> 6][lane >> 4][rs8_index(lane)] = x;
@@ -1533,20 +1540,23 @@ void launch_sweep_k(dim3 g, dim3 b, hipStream_t st, const Ctx &c, int colour, un
                      c.SX, c.SY, c.SZ, c.surf_d, c.spart, fin, c.fused ? c.nlat : 0, c.sr_d, c.mom_d, c.obs_d);
 }
 
+// meas (the measurement folded into the colour-1 pass) is never set for pear contexts: they always measure the pear
+// observables, which keep the separate measurement kernels, so no pear measuring instance is compiled.
 template <int POT, bool MASK, bool SURF, bool FIELD, bool FLIP, bool FREE>
 void launch_sweep(dim3 g, dim3 b, hipStream_t st, const Ctx &c, int colour, unsigned k, bool meas, int fin) {
-  if (c.row) {
-    if (meas)
-      launch_sweep_k<POT, MASK, SURF, FIELD, FLIP, FREE, true, true>(g, b, st, c, colour, k, fin);
-    else
-      launch_sweep_k<POT, MASK, SURF, FIELD, FLIP, FREE, false, true>(g, b, st, c, colour, k, fin);
-    return;
+  if constexpr (POT != MCL_POT_PEAR) {
+    if (meas) {
+      if (c.row)
+        launch_sweep_k<POT, MASK, SURF, FIELD, FLIP, FREE, true, true>(g, b, st, c, colour, k, fin);
+      else
+        launch_sweep_k<POT, MASK, SURF, FIELD, FLIP, FREE, true, false>(g, b, st, c, colour, k, fin);
+      return;
+    }
   }
-  if (meas) {
-    launch_sweep_k<POT, MASK, SURF, FIELD, FLIP, FREE, true, false>(g, b, st, c, colour, k, fin);
-    return;
-  }
-  launch_sweep_k<POT, MASK, SURF, FIELD, FLIP, FREE, false, false>(g, b, st, c, colour, k, fin);
+  if (c.row)
+    launch_sweep_k<POT, MASK, SURF, FIELD, FLIP, FREE, false, true>(g, b, st, c, colour, k, fin);
+  else
+    launch_sweep_k<POT, MASK, SURF, FIELD, FLIP, FREE, false, false>(g, b, st, c, colour, k, fin);
 }
 
 // Grid of a sweep launch: the work list, plus one workgroup per lattice for the deferred finalize of a measurement
@@ -2149,13 +2159,24 @@ static int enqueue_sweeps(Ctx *c, SweepFn sf, int k0, int n, int measure_every) 
   return flush_finalize(c, &pending);
 }
 
-// Graph of `len` sweeps (with the measurements of measure_every) followed by the advance of the sweep base.
+// Captured graphs a context keeps; the least recently replayed one is dropped beyond this.
+constexpr size_t kMaxGraphs = 8;
+
+// Graph of `len` sweeps (with the measurements of measure_every) followed by the advance of the sweep base, from the
+// context's cache (most recently used last) or captured now.
 static int get_graph(Ctx *c, SweepFn sf, int len, int measure_every, hipGraphExec_t *out) {
-  for (auto &g : c->graphs)
-    if (g.first.first == len && g.first.second == measure_every) {
-      *out = g.second;
+  for (size_t i = 0; i < c->graphs.size(); i++)
+    if (c->graphs[i].first.first == len && c->graphs[i].first.second == measure_every) {
+      std::rotate(c->graphs.begin() + i, c->graphs.begin() + i + 1, c->graphs.end());
+      *out = c->graphs.back().second;
       return MCL_OK;
     }
+  if (c->graphs.size() >= kMaxGraphs) {
+    // the oldest graph may still be queued on the stream: its replays end before it is destroyed
+    MCL_HIP(hipStreamSynchronize(c->stream));
+    (void)hipGraphExecDestroy(c->graphs.front().second);
+    c->graphs.erase(c->graphs.begin());
+  }
   hipGraph_t graph = nullptr;
   MCL_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
   int st = enqueue_sweeps(c, sf, 0, len, measure_every);
@@ -2175,12 +2196,66 @@ static int get_graph(Ctx *c, SweepFn sf, int len, int measure_every, hipGraphExe
   return MCL_OK;
 }
 
+// How mcl_sweeps splits a call of nsweeps when graphs are on: nblocks replays of a graph of `block` sweeps (the graph
+// block rounded to a multiple of measure_every), then one replay of a graph of `tail` sweeps (the largest multiple of
+// measure_every in the rest, or the whole rest when nothing is measured); what remains (fewer than measure_every
+// sweeps, none of them measured) runs eagerly. Every graph length is a multiple of measure_every, so the
+// measurements fall where the eager path puts them.
+struct GraphPlan {
+  int block = 0, nblocks = 0, tail = 0;
+};
+
+static GraphPlan graph_plan(const Ctx *c, int nsweeps, int measure_every) {
+  GraphPlan g;
+  if (c->graph_block <= 0) return g;
+  g.block = c->graph_block;
+  if (measure_every > 0) g.block = measure_every * std::max(1, (c->graph_block + measure_every / 2) / measure_every);
+  g.nblocks = nsweeps / g.block;
+  const int rest = nsweeps - g.nblocks * g.block;
+  g.tail = measure_every > 0 ? rest - rest % measure_every : rest;
+  return g;
+}
+
+// Whether plain sweeps of this call run as resident launches (small lattices, no measurement, no kernel timing).
+static bool runs_resident(const Ctx *c, int measure_every) {
+  return measure_every == 0 && c->resident_ok && c->resident_on && c->timing_stride == 0;
+}
+
+// Replays graph g of len sweeps; with kernel timing on, one event pair around the replay.
+static int replay(Ctx *c, hipGraphExec_t g, int len) {
+  const bool timed = c->timing_stride > 0 && c->ev_used + 2 <= c->ev.size();
+  if (timed) MCL_HIP(hipEventRecord(c->ev[c->ev_used], c->stream));
+  MCL_HIP(hipGraphLaunch(g, c->stream));
+  if (timed) {
+    MCL_HIP(hipEventRecord(c->ev[c->ev_used + 1], c->stream));
+    c->ev_runs.push_back({c->ev_used, 2 * len});
+    c->ev_used += 2;
+  }
+  c->sweeps_done += (unsigned long long)len;
+  return MCL_OK;
+}
+
+int mcl_prepare_sweeps(void *ctx, int32_t nsweeps, int32_t measure_every) {
+  Ctx *c = as_ctx(ctx);
+  if (!c) return fail(MCL_ERR_ARG, "null context");
+  if (nsweeps < 0 || measure_every < 0) return fail(MCL_ERR_ARG, "negative sweep count");
+  if (nsweeps == 0 || runs_resident(c, measure_every)) return MCL_OK;
+  const SweepFn sf = pick_sweep(*c);
+  const GraphPlan plan = graph_plan(c, nsweeps, measure_every);
+  hipGraphExec_t g;
+  if (plan.nblocks > 0)
+    if (int st = get_graph(c, sf, plan.block, measure_every, &g)) return st;
+  if (plan.tail > 0)
+    if (int st = get_graph(c, sf, plan.tail, measure_every, &g)) return st;
+  return MCL_OK;
+}
+
 int mcl_sweeps(void *ctx, int32_t nsweeps, int32_t measure_every) {
   Ctx *c = as_ctx(ctx);
   if (!c) return fail(MCL_ERR_ARG, "null context");
   if (nsweeps < 0 || measure_every < 0) return fail(MCL_ERR_ARG, "negative sweep count");
   if (nsweeps == 0) return MCL_OK;
-  if (measure_every == 0 && c->resident_ok && c->resident_on && c->timing_stride == 0) {
+  if (runs_resident(c, measure_every)) {
     // plain sweeps of small lattices: whole sweeps in one launch per kResChunk, bit-identical to the passes below
     const ResFn rf = pick_resident(*c);
     for (int k0 = 0; k0 < nsweeps; k0 += kResChunk) rf(c->stream, *c, (unsigned)k0, std::min(kResChunk, nsweeps - k0));
@@ -2192,27 +2267,18 @@ int mcl_sweeps(void *ctx, int32_t nsweeps, int32_t measure_every) {
   const SweepFn sf = pick_sweep(*c);
   const dim3 grid = sweep_grid(c), block(c->bs);
   int done = 0;
-  if (c->graph_block > 0) {
-    // whole graphs of len sweeps (a multiple of measure_every, so the measurements fall where the eager path
-    // puts them), then the rest eagerly; with kernel timing on, one event pair around each replay
-    int len = c->graph_block;
-    if (measure_every > 0) len = measure_every * std::max(1, (c->graph_block + measure_every / 2) / measure_every);
-    if (nsweeps >= len) {
-      hipGraphExec_t g;
-      int st = get_graph(c, sf, len, measure_every, &g);
-      if (st) return st;
-      for (; done + len <= nsweeps; done += len) {
-        const bool timed = c->timing_stride > 0 && c->ev_used + 2 <= c->ev.size();
-        if (timed) MCL_HIP(hipEventRecord(c->ev[c->ev_used], c->stream));
-        MCL_HIP(hipGraphLaunch(g, c->stream));
-        if (timed) {
-          MCL_HIP(hipEventRecord(c->ev[c->ev_used + 1], c->stream));
-          c->ev_runs.push_back({c->ev_used, 2 * len});
-          c->ev_used += 2;
-        }
-      }
-      c->sweeps_done += (unsigned long long)done;
-    }
+  const GraphPlan plan = graph_plan(c, nsweeps, measure_every);
+  if (plan.nblocks > 0) {
+    hipGraphExec_t g;
+    if (int st = get_graph(c, sf, plan.block, measure_every, &g)) return st;
+    for (int b = 0; b < plan.nblocks; b++, done += plan.block)
+      if (int st = replay(c, g, plan.block)) return st;
+  }
+  if (plan.tail > 0) {
+    hipGraphExec_t g;
+    if (int st = get_graph(c, sf, plan.tail, measure_every, &g)) return st;
+    if (int st = replay(c, g, plan.tail)) return st;
+    done += plan.tail;
   }
   const int rest = nsweeps - done;
   if (rest > 0) {
@@ -2626,8 +2692,12 @@ std::string tune_key(const Ctx *c) {
   while ((1ll << lp) < c->pool) lp++;
   while ((1 << lm) < maxnc) lm++;
   char buf[160];
-  snprintf(buf, sizeof(buf), "sweep:pot%d:mask%d:surf%d:field%d:flip%d:nlat%d:pool2^%d:maxnc2^%d", c->pot, c->mask,
-           c->surf, c->field, c->flip, c->nlat, lp, lm);
+  // every flag that selects the kernel instance (so contexts that run different kernels do not share a result),
+  // then the sizes
+  snprintf(buf, sizeof(buf),
+           "sweep:pot%d:mask%d:surf%d:field%d:flip%d:free%d:row%d:split%d:fused%d:empty%d:xq%d:nlat%d:pool2^%d:maxnc2^%d",
+           c->pot, c->mask, c->surf, c->field, c->flip, c->free, c->row, c->split, c->fused, c->has_empty, c->xq,
+           c->nlat, lp, lm);
   return buf;
 }
 

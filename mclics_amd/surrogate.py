@@ -9,7 +9,7 @@ then used to
 Every point is afterwards run with the plain method, which produces every reported number and measures the
 surrogate's error (tni.py). The Metropolis rule itself is never changed.
 
-    python -m mclics_amd.surrogate generate --runs 25 --out data.npz     # batched plain runs -> pairs
+    python -m mclics_amd.surrogate generate --runs 25 --out data.npz     # batched plain runs -> pairs (12^3-20^3)
     python -m mclics_amd.surrogate train data.npz --out net.npz           # HIP training
 """
 import argparse
@@ -33,9 +33,11 @@ def log(m):
 # ---------------------------------------------------------------------------------------------------------------
 # Training data from batched plain Metropolis runs
 
-def random_lattices(n, rng, lmin=16, lmax=64):
-    """Lattices of the training distribution, the batched configuration widened over the network's inputs: size L
-    uniform over even integers, bulk (W = 0) or homeotropic sphere (W uniform in [0.5, 1.5]) with probability 1/2,
+def random_lattices(n, rng, lmin=12, lmax=20):
+    """Lattices of the training distribution, the batched configuration's physics widened over the network's inputs:
+    size L uniform over even integers (small lattices: the network only places points and warm-starts, and a query
+    beyond the largest trained size is asked at that size), bulk (W = 0) or homeotropic sphere (W uniform in
+    [0.5, 1.5]) with probability 1/2,
     nematic coupling A uniform in [0.8, 1.2], dielectric coupling along z uniform in [0, 0.3] on half the lattices,
     and a ladder direction: cooling from random directors or heating from directors along z."""
     out = []
@@ -55,7 +57,7 @@ def features(T, A, W, field, nvalid):
                      np.full(T.shape, field, dtype=np.float64), np.full(T.shape, np.log(nvalid))], axis=-1)
 
 
-def generate(runs=25, lattices=512, rungs=16, mct=250, mcs=750, every=3, seed=1, lmin=16, lmax=64, tmin=0.9,
+def generate(runs=25, lattices=512, rungs=16, mct=150, mcs=450, every=3, seed=1, lmin=12, lmax=20, tmin=0.9,
              tmax=1.3, log=log):
     """Pairs (features, observables) from `runs` batched runs. Each lattice walks a ladder of `rungs` temperatures
     drawn uniformly in [tmin A, tmax A] (downwards from random directors or upwards from directors along z),
@@ -227,8 +229,10 @@ def main(argv=None):
     g.add_argument("--runs", type=int, default=25)
     g.add_argument("--lattices", type=int, default=512)
     g.add_argument("--rungs", type=int, default=16)
-    g.add_argument("--mct", type=int, default=250)
-    g.add_argument("--mcs", type=int, default=750)
+    g.add_argument("--mct", type=int, default=150)
+    g.add_argument("--mcs", type=int, default=450)
+    g.add_argument("--lmin", type=int, default=12)
+    g.add_argument("--lmax", type=int, default=20)
     g.add_argument("--every", type=int, default=3)
     g.add_argument("--seed", type=int, default=1)
     g.add_argument("--out", required=True)
@@ -238,7 +242,7 @@ def main(argv=None):
     t.add_argument("--out", required=True)
     a = ap.parse_args(argv)
     if a.cmd == "generate":
-        X, Y, upd = generate(a.runs, a.lattices, a.rungs, a.mct, a.mcs, a.every, a.seed)
+        X, Y, upd = generate(a.runs, a.lattices, a.rungs, a.mct, a.mcs, a.every, a.seed, a.lmin, a.lmax)
         np.savez_compressed(a.out, X=X, Y=Y, site_updates=upd)
         print(json.dumps({"pairs": len(X), "site_updates": upd}))
     else:

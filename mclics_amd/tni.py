@@ -294,6 +294,13 @@ def experiment(net, L=96, seeds=(1, 2, 3, 4), dTs=(0.004, 0.002, 0.001, 0.0005),
     g["within"] = within(g["estimates"], tol)
     out["guided"] = g
     log("guided: %s (%.1f s)" % (json.dumps(g), g["seconds"]))
+    # the same window at half the points (spacing 0.002): the placement alone must then carry the estimate
+    t0 = time.perf_counter()
+    g16 = guided_scan(sur, L, seeds, n=16, spacing=2e-3, **run)
+    g16["seconds"] = time.perf_counter() - t0
+    g16["within"] = within(g16["estimates"], tol)
+    out["guided_16"] = g16
+    log("guided, 16 points: %s (%.1f s)" % (json.dumps(g16), g16["seconds"]))
     mct, mcs = npp // 5, npp - npp // 5
     out["uniform"] = []
     for dT in dTs:
@@ -321,10 +328,15 @@ def experiment(net, L=96, seeds=(1, 2, 3, 4), dTs=(0.004, 0.002, 0.001, 0.0005),
     else:
         out["ratio_same_sweeps_per_point"] = base["sweeps"] / gs
     out["ratio_reference_protocol"] = sweeps / gs if ref["within"] and g["within"] else None
+    gs16 = max(g16["sweeps"])
+    out["guided_16_sweeps"] = gs16
+    out["ratio_reference_protocol_16"] = sweeps / gs16 if ref["within"] and g16["within"] else None
     if data_updates is not None:
         eq = data_updates / float(L) ** 3
         out["training_sweeps_equivalent"] = eq
         out["ratio_reference_protocol_with_training"] = sweeps / (gs + eq) if ref["within"] and g["within"] else None
+        out["ratio_reference_protocol_16_with_training"] = (sweeps / (gs16 + eq) if ref["within"] and g16["within"]
+                                                            else None)
     if ablations:
         t0 = time.perf_counter()
         cold = guided_scan(sur, L, seeds, warm=False, **run)
@@ -338,7 +350,7 @@ def experiment(net, L=96, seeds=(1, 2, 3, 4), dTs=(0.004, 0.002, 0.001, 0.0005),
         out["guided_untrained"] = un
         log("guided with an untrained network: %s (%.1f s)" % (json.dumps(un), time.perf_counter() - t0))
         out["uniform_coexistence"] = []
-        for n in (32, 56, 111):
+        for n in (16, 32, 40, 48, 56, 111):
             t0 = time.perf_counter()
             grid = np.linspace(1.02, 1.24, n)
             est, sweeps = uniform_coexistence(L, grid, seeds, **run)
@@ -346,6 +358,13 @@ def experiment(net, L=96, seeds=(1, 2, 3, 4), dTs=(0.004, 0.002, 0.001, 0.0005),
                  "within": within(est, tol), "seconds": time.perf_counter() - t0}
             out["uniform_coexistence"].append(u)
             log("coexistence on a uniform grid: %s" % json.dumps(u))
+        # the network's own gain: the fewest uniform points at which the same estimator meets the tolerance for
+        # every seed, against the guided scan's points
+        ok = [u for u in out["uniform_coexistence"] if u["within"]]
+        if ok:
+            least = min(ok, key=lambda u: u["points"])
+            out["network_gain_points_32"] = least["points"] / 32.0 if g["within"] else None
+            out["network_gain_points_16"] = least["points"] / 16.0 if g16["within"] else None
     return out
 
 

@@ -57,6 +57,40 @@ def test_graph_replay_matches_eager(mcl):
         assert same(a, b), "call %d differs" % k
 
 
+BULK_SPECS = [dict(L=24, geometry="bulk", T=1.05, seed=21), dict(L=16, geometry="bulk", T=0.8, seed=22)]
+
+
+@pytest.mark.parametrize("specs", [SPECS, BULK_SPECS], ids=["mixed", "bulk-fused"])
+def test_short_calls_replay_graphs_like_eager(mcl, specs):
+    """With a 50-sweep graph block, calls shorter than the block replay a graph of their own length (the largest
+    multiple of measure_every; the rest eagerly), calls longer than it add such a tail after the whole blocks, and
+    mcl_prepare_sweeps captures them ahead without running anything. More distinct lengths than the context's 8
+    cached graphs evict the least recently used ones. Every call's state equals the eager run's. The bulk-only
+    context measures inside the sweep kernel (its finalize then ends each graph)."""
+    calls = ((20, 10), (5, 0), (13, 4), (9, 3), (11, 0), (22, 2), (3, 1), (17, 0), (6, 6), (20, 10), (14, 7),
+             (1, 0), (120, 10), (20, 10))
+    prepared = {0, 6, 12}
+
+    def go(graph):
+        with mcl.Engine([batch.build_lattice(s) for s in specs]) as eng:
+            if graph:
+                eng.set_graph(graph)
+            out = []
+            for k, (n, every) in enumerate(calls):
+                if k in prepared:
+                    before = state(eng, len(specs))
+                    eng.prepare(n, every)
+                    assert same(before, state(eng, len(specs))), "prepare changed the state"
+                eng.sweeps(n, every)
+                out.append(state(eng, len(specs)))
+            return out
+
+    eager, graph = go(0), go(50)
+    for k, (a, b) in enumerate(zip(eager, graph)):
+        assert same(a, b), "call %d %s differs" % (k, calls[k])
+    assert graph[-1][3][0][1] == sum(n for n, _ in calls)
+
+
 def test_batched_lattice_equals_lattice_alone(mcl):
     with mcl.Engine([batch.build_lattice(s) for s in SPECS]) as eng:
         eng.sweeps(25, 5)
