@@ -143,3 +143,52 @@ def test_small_batch_physics(mcl):
     assert np.all(S[bulk & (T > 1.5)] < 0.4)
     assert np.all(rows[:, 2] >= 0) and np.all(rows[:, 4] >= 0) and np.all(np.isfinite(rows))
     assert info["site_updates_per_ns"] > 0
+
+
+def test_c4_batch_of_512_lattices(mcl, orc):
+    """The batched configuration at full size: 512 LL lattices of 16^3-64^3, bulk or homeotropic sphere (W = 1.0),
+    T uniform in [0.9, 1.3], one context with the autotuner's launch shape (trials at the run's cadence) and graph
+    replay; MCT = 300, then MCS = 300 measured every 10. Every lattice gets 30 measurements with 0 < S < 1 and a mean
+    site energy between the ground state and the disordered value; the bulk lattices below T = 1.0 are ordering (mean
+    S above 0.1 and four times that of the bulk lattices above 1.2). For the smallest and the largest lattice and one sphere: the FP64 energy measured in the
+    batch equals the oracle's energy of the same final directors (1e-6), and the lattice run alone (same seed and
+    sweeps) ends with the same directors, bit for bit."""
+    specs = batch.c4_specs(512, 1)
+    sizes = np.array([s["L"] for s in specs])
+    spheres = [r for r, s in enumerate(specs) if s["geometry"] == "sphere"]
+    picks = sorted({int(np.argmin(sizes)), int(np.argmax(sizes)), spheres[len(spheres) // 2]})
+
+    with mcl.Engine([batch.build_lattice(s) for s in specs]) as eng:
+        eng.autotune(10, 10)
+        eng.set_graph(32)
+        eng.sweeps(300)
+        eng.reset_moments()
+        eng.sweeps(300, 10)
+        moms = [eng.moments(r) for r in range(len(specs))]
+        finals = {r: eng.directors(r) for r in picks}
+        measured = {r: eng.measure(r)["E"] for r in picks}
+    S = np.array([m["S"] / m["n"] for m in moms])
+    E = np.array([m["E"] / m["n"] for m in moms])
+    assert all(m["n"] == 30 for m in moms)
+    assert np.all((S > 0) & (S < 1)) and np.all((E > -10.0) & (E < -1.0))
+    T = np.array([s["T"] for s in specs])
+    bulk = np.array([s["geometry"] == "bulk" for s in specs])
+    # from random directors, 300 sweeps order the cold lattices only partly (domains coarsen slowly at 64^3)
+    cold, hot = S[bulk & (T < 1.0)].mean(), S[bulk & (T > 1.2)].mean()
+    assert cold > 0.1 and cold > 4 * hot, (cold, hot)
+    for r in picks:
+        s = specs[r]
+        L = s["L"]
+        if s["geometry"] == "sphere":
+            pt, ns = batch.sphere_geometry(L)
+            p = orc.params(L, L, L, periodic=(0, 0, 0), potential="ll", A=1.0,
+                           surfaces=[("homeotropic", 1.0, (0, 0, 0))])
+            e = orc.energy_mean(p, finals[r], np.array(pt, dtype=np.int32), np.array(ns, dtype=np.float32))
+        else:
+            p = orc.params(L, L, L, periodic=(1, 1, 1), potential="ll", A=1.0)
+            e = orc.energy_mean(p, finals[r], np.ones(L ** 3, dtype=np.int32))
+        assert measured[r] == pytest.approx(e, rel=1e-6), (r, s)
+        with mcl.Engine(batch.build_lattice(s)) as eng:
+            eng.sweeps(300)
+            eng.sweeps(300, 10)
+            assert np.array_equal(eng.directors(), finals[r]), (r, s)
