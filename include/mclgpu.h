@@ -158,6 +158,14 @@ int mcl_abi_version(void);
  * in MC_GPU, evolve.cu:205-209). */
 int mcl_site_energies(void *ctx, int32_t lat, const int32_t *sites, const float *trial, int32_t n, float *e_cur,
                       float *e_trial);
+/* No counterpart in the reference. mcl_site_energies without any device allocation, for callers that cannot
+ * allocate: code under HIP graph capture, tight loops of proposals, harnesses that forbid hipMalloc. The staging
+ * of at most MCL_SCRATCH_SITES sites per call (24 B each, 1.5 MiB) lives in one static device array per GPU, held
+ * until the process ends; larger n is rejected with MCL_ERR_ARG. Calls that overlap on one GPU share that array and
+ * race. The results equal mcl_site_energies' bit for bit. */
+#define MCL_SCRATCH_SITES 65536
+int mcl_site_energies_noalloc(void *ctx, int32_t lat, const int32_t *sites, const float *trial, int32_t n,
+                              float *e_cur, float *e_trial);
 /* Philox4x32-10 of the device generator: out[4k..4k+3] = philox(ctr[4k..4k+3], key[2k..2k+1]). */
 int mcl_philox(const uint32_t *ctr, const uint32_t *key, int32_t n, uint32_t *out);
 

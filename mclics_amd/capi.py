@@ -89,6 +89,9 @@ ABI = {
     "mcl_site_energies": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32),
                                          ctypes.POINTER(ctypes.c_float), ctypes.c_int32,
                                          ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]),
+    "mcl_site_energies_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32),
+                                                 ctypes.POINTER(ctypes.c_float), ctypes.c_int32,
+                                                 ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]),
     "mcl_philox": (ctypes.c_int, [ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32), ctypes.c_int32,
                                   ctypes.POINTER(ctypes.c_uint32)]),
     "mcl_set_kernel_timing": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]),
@@ -305,6 +308,17 @@ class Engine:
         et = np.empty(len(s), dtype=np.float32)
         _check(self.lib.mcl_site_energies(self.h, lat, s.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), _fptr(t),
                                           len(s), _fptr(ec), _fptr(et)))
+        return ec, et
+
+    def site_energies_noalloc(self, sites, trial, lat=0):
+        """mcl_site_energies_noalloc: the same energies without any device allocation (at most MCL_SCRATCH_SITES
+        sites per call)."""
+        s = np.ascontiguousarray(sites, dtype=np.int32)
+        t = np.ascontiguousarray(trial, dtype=np.float32).reshape(-1)
+        ec = np.empty(len(s), dtype=np.float32)
+        et = np.empty(len(s), dtype=np.float32)
+        _check(self.lib.mcl_site_energies_noalloc(self.h, lat, s.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                                  _fptr(t), len(s), _fptr(ec), _fptr(et)))
         return ec, et
 
     def autotune(self, trial_sweeps=20, measure_every=0):

@@ -30,6 +30,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1726,6 +1727,35 @@ void destroy_ctx(Ctx *c) {
   delete c;
 }
 
+// Staging of mcl_site_energies_noalloc: MCL_SCRATCH_SITES site indices, trial directors and the two energies (24 B per
+// site, 1.5 MiB) in one file-scope device array. Each GPU the library runs on holds its own copy until the process
+// ends; every call on a GPU uses that GPU's copy, so calls that overlap on one GPU race. On the host the name is only
+// a stand-in for the array: its device address comes from hipGetSymbolAddress, looked up once per device (with that
+// device current) when the first context is created there, and only that address is handed to copies and kernels.
+constexpr size_t kScratchBytes = (size_t)MCL_SCRATCH_SITES * (sizeof(int) + 5 * sizeof(float));
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+static std::mutex g_scratch_mu;
+static std::vector<void *> g_scratch_dev;  // device address of `scratch`, by device index; nullptr until looked up
+
+// Looks up the current device's address of `scratch` (device index `device`) unless it is known.
+static int scratch_lookup(int device) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if ((int)g_scratch_dev.size() <= device) g_scratch_dev.resize((size_t)device + 1, nullptr);
+  if (g_scratch_dev[device]) return MCL_OK;
+  void *ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr)
+    return fail(MCL_ERR_DEVICE, std::string("scratch address: ") + hipGetErrorString(e));
+  g_scratch_dev[device] = ptr;
+  return MCL_OK;
+}
+
+static void *scratch_address(int device) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  return device >= 0 && device < (int)g_scratch_dev.size() ? g_scratch_dev[device] : nullptr;
+}
+
 template <typename T>
 int dev_alloc(T **p, size_t n) {
   void *q = nullptr;
@@ -1778,6 +1808,7 @@ int build_worklist(Ctx *c, int bs, int xcd) {
 int create_ctx(const mcl_lattice_desc *lat, int nlat, Ctx *c) {
   if (!lat || nlat < 1) return fail(MCL_ERR_ARG, "no lattices");
   MCL_HIP(hipGetDevice(&c->device));
+  if (int st = scratch_lookup(c->device)) return st;
   MCL_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   c->nlat = nlat;
   c->pot = lat[0].model.potential;
@@ -2545,9 +2576,9 @@ int mcl_synchronize(void *ctx) {
   return MCL_OK;
 }
 
-int mcl_site_energies(void *ctx, int32_t lat, const int32_t *sites, const float *trial, int32_t n, float *e_cur,
-                      float *e_trial) {
-  Ctx *c = as_ctx(ctx);
+// Checks of the site-energy calls: the lattice, the pointers and every site index.
+static int check_site_query(Ctx *c, int32_t lat, const int32_t *sites, const float *trial, int32_t n,
+                            const float *e_cur, const float *e_trial) {
   int st = check_lat(c, lat);
   if (st) return st;
   if (n <= 0) return MCL_OK;
@@ -2556,16 +2587,13 @@ int mcl_site_energies(void *ctx, int32_t lat, const int32_t *sites, const float 
   const long long Nt = (long long)h.nx * h.ny * h.nz;
   for (int g = 0; g < n; g++)
     if (sites[g] < 0 || sites[g] >= Nt) return fail(MCL_ERR_ARG, "site index out of range");
-  int *d_sites = nullptr;
-  float *d_trial = nullptr, *d_ec = nullptr, *d_et = nullptr;
-  if ((st = dev_alloc(&d_sites, n)) || (st = dev_alloc(&d_trial, 3 * (size_t)n)) || (st = dev_alloc(&d_ec, n)) ||
-      (st = dev_alloc(&d_et, n))) {
-    (void)hipFree(d_sites);
-    (void)hipFree(d_trial);
-    (void)hipFree(d_ec);
-    (void)hipFree(d_et);
-    return st;
-  }
+  return MCL_OK;
+}
+
+// Site energies through device staging buffers: copies in, the sweep's energy function, copies out, wait.
+static hipError_t site_energies_staged(Ctx *c, int32_t lat, const int32_t *sites, const float *trial, int32_t n,
+                                       float *e_cur, float *e_trial, int *d_sites, float *d_trial, float *d_ec,
+                                       float *d_et) {
   hipError_t e = hipMemcpyAsync(d_sites, sites, sizeof(int) * n, hipMemcpyHostToDevice, c->stream);
   if (e == hipSuccess)
     e = hipMemcpyAsync(d_trial, trial, sizeof(float) * 3 * n, hipMemcpyHostToDevice, c->stream);
@@ -2580,10 +2608,49 @@ int mcl_site_energies(void *ctx, int32_t lat, const int32_t *sites, const float 
   if (e == hipSuccess) e = hipMemcpyAsync(e_cur, d_ec, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream);
   if (e == hipSuccess) e = hipMemcpyAsync(e_trial, d_et, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  return e;
+}
+
+int mcl_site_energies(void *ctx, int32_t lat, const int32_t *sites, const float *trial, int32_t n, float *e_cur,
+                      float *e_trial) {
+  Ctx *c = as_ctx(ctx);
+  int st = check_site_query(c, lat, sites, trial, n, e_cur, e_trial);
+  if (st || n <= 0) return st;
+  int *d_sites = nullptr;
+  float *d_trial = nullptr, *d_ec = nullptr, *d_et = nullptr;
+  if ((st = dev_alloc(&d_sites, n)) || (st = dev_alloc(&d_trial, 3 * (size_t)n)) || (st = dev_alloc(&d_ec, n)) ||
+      (st = dev_alloc(&d_et, n))) {
+    (void)hipFree(d_sites);
+    (void)hipFree(d_trial);
+    (void)hipFree(d_ec);
+    (void)hipFree(d_et);
+    return st;
+  }
+  const hipError_t e = site_energies_staged(c, lat, sites, trial, n, e_cur, e_trial, d_sites, d_trial, d_ec, d_et);
   (void)hipFree(d_sites);
   (void)hipFree(d_trial);
   (void)hipFree(d_ec);
   (void)hipFree(d_et);
+  if (e != hipSuccess) return fail(MCL_ERR_DEVICE, std::string("site energies: ") + hipGetErrorString(e));
+  return MCL_OK;
+}
+
+int mcl_site_energies_noalloc(void *ctx, int32_t lat, const int32_t *sites, const float *trial, int32_t n,
+                              float *e_cur, float *e_trial) {
+  Ctx *c = as_ctx(ctx);
+  if (c && n > MCL_SCRATCH_SITES)
+    return fail(MCL_ERR_ARG, "site energies without allocation: " + std::to_string(n) + " sites requested, at most " +
+                                 std::to_string(MCL_SCRATCH_SITES) + " per call");
+  int st = check_site_query(c, lat, sites, trial, n, e_cur, e_trial);
+  if (st || n <= 0) return st;
+  unsigned char *base = static_cast<unsigned char *>(scratch_address(c->device));
+  if (!base) return fail(MCL_ERR_DEVICE, "site energies without allocation: no scratch address for this device");
+  // [site indices | trial directors | current energies | trial energies], each sized for MCL_SCRATCH_SITES
+  int *d_sites = reinterpret_cast<int *>(base);
+  float *d_trial = reinterpret_cast<float *>(base + sizeof(int) * (size_t)MCL_SCRATCH_SITES);
+  float *d_ec = d_trial + 3 * (size_t)MCL_SCRATCH_SITES;
+  float *d_et = d_ec + MCL_SCRATCH_SITES;
+  const hipError_t e = site_energies_staged(c, lat, sites, trial, n, e_cur, e_trial, d_sites, d_trial, d_ec, d_et);
   if (e != hipSuccess) return fail(MCL_ERR_DEVICE, std::string("site energies: ") + hipGetErrorString(e));
   return MCL_OK;
 }
