@@ -601,27 +601,36 @@ __device__ __forceinline__ void finalize_compact(int r, const double *__restrict
   // virtual wave vw adding partials rg.x + 64 vw + l, + kFinBlock, ...; each virtual wave summed by wave_sum; the
   // wave sums added in order. A measured sweep then gets the same moments, bit for bit, whether its finalize runs
   // here (in the next colour-0 pass) or in k_finalize (after the last sweep of a call or of a captured graph).
+  // The (virtual wave, quantity) pairs are spread over the workgroup's waves, one accumulator each: the finalize is
+  // inlined into every sweep instance, whose plain passes are held to 64 VGPRs. A lane loads four of its partials
+  // at a time and adds them in increasing order, so the one finalizing workgroup waits for few memory round trips
+  // and ends well within the colour pass it runs beside.
   __shared__ double fred[kFinWaves][kNQS];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = (int)(blockDim.x >> 6);
-  for (int vw = w; vw < kFinWaves; vw += nw) {
-    double v[kNQS];
+#pragma unroll 1
+  for (int j = w; j < kFinWaves * kNQS; j += nw) {
+    const int vw = j / kNQS, a = j - vw * kNQS;
+    const double *__restrict__ col = part + (size_t)a * stride;
+    double v = 0.0;
+#pragma unroll 1
+    for (int p = rg.x + 64 * vw + lane; p < rg.y; p += 4 * kFinBlock) {
+      double x[4];
 #pragma unroll
-    for (int a = 0; a < kNQS; a++) v[a] = 0.0;
-    for (int p = rg.x + 64 * vw + lane; p < rg.y; p += kFinBlock) {
+      for (int u = 0; u < 4; u++) x[u] = p + u * kFinBlock < rg.y ? col[p + u * kFinBlock] : 0.0;
 #pragma unroll
-      for (int a = 0; a < kNQS; a++) v[a] += part[(size_t)a * stride + p];
+      for (int u = 0; u < 4; u++)
+        if (p + u * kFinBlock < rg.y) v += x[u];
     }
-#pragma unroll
-    for (int a = 0; a < kNQS; a++) {
-      const double x = wave_sum(v[a]);
-      if (lane == 0) fred[vw][a] = x;
-    }
+    const double x = wave_sum(v);
+    if (lane == 0) fred[vw][a] = x;
   }
   __syncthreads();
   if (threadIdx.x != 0) return;
   double t[kNQS];
+#pragma unroll
   for (int a = 0; a < kNQS; a++) {
     double x = 0.0;
+#pragma unroll 1
     for (int ww = 0; ww < kFinWaves; ww++) x += fred[ww][a];
     t[a] = x;
   }
