@@ -175,7 +175,8 @@ int mcl_philox(const uint32_t *ctr, const uint32_t *key, int32_t n, uint32_t *ou
  * sweeps. With graph replay on (mcl_set_graph), the pair goes around each replay instead, and the replay's duration
  * counts for its 2 * len sweep-kernel launches (including the measurement work folded into them and the replay's
  * final finalize). mcl_kernel_timing returns the number of timed sweep-kernel launches and their summed duration
- * (synchronizes). stride = 0 switches timing off. */
+ * (synchronizes); with lattice groups (mcl_set_lattice_groups) a launch over one group counts as its share of the
+ * context's colour-block positions, so the count is in whole-context colour passes. stride = 0 switches timing off. */
 int mcl_set_kernel_timing(void *ctx, int32_t stride, int32_t max_timed_sweeps);
 int mcl_kernel_timing(void *ctx, int64_t *launches, double *total_ms);
 /* STREAM-style copy of nbytes (rounded down to 16 B) between two fresh device buffers, reps times on the null
@@ -224,6 +225,16 @@ int mcl_prepare_sweeps(void *ctx, int32_t nsweeps, int32_t measure_every);
 int mcl_set_resident(void *ctx, int32_t enable);
 /* *active = 1 when plain sweeps of the context run resident (the context qualifies and it is switched on). */
 int mcl_resident(void *ctx, int32_t *active);
+/* Lattice groups (no reference counterpart; for batches larger than the GPU's 256 MB Infinity Cache). When the
+ * directors and point types of the context's lattices, both colours, exceed max_bytes, mcl_sweeps splits the
+ * lattices into groups of consecutive lattices of about equal size, each within max_bytes, and runs every sweep of
+ * the call for one group before the next, so that each group's colour passes are served from that cache instead of
+ * HBM. Results are identical to the ungrouped order (a lattice's trajectory depends only on its own sites and sweep
+ * numbers); measurements, graphs and kernel timing cover every group. Default 192 MiB; max_bytes <= 0 keeps one
+ * group. Drops the context's captured graphs. */
+int mcl_set_lattice_groups(void *ctx, int64_t max_bytes);
+/* *ngroups = the number of lattice groups mcl_sweeps runs in turn (1 when the context fits). */
+int mcl_get_lattice_groups(void *ctx, int32_t *ngroups);
 
 /* Device stream of the context (a hipStream_t), for callers that record their own events. */
 int mcl_get_stream(void *ctx, void **stream);

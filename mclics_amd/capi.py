@@ -105,6 +105,8 @@ ABI = {
     "mcl_prepare_sweeps": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]),
     "mcl_set_resident": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "mcl_resident": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32)]),
+    "mcl_set_lattice_groups": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64]),
+    "mcl_get_lattice_groups": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32)]),
     "mcl_mlp_create": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_uint64,
                                       ctypes.POINTER(ctypes.c_void_p)]),
     "mcl_mlp_destroy": (None, [ctypes.c_void_p]),
@@ -346,6 +348,16 @@ class Engine:
         v = ctypes.c_int32()
         _check(self.lib.mcl_resident(self.h, ctypes.byref(v)))
         return bool(v.value)
+
+    def set_lattice_groups(self, max_bytes):
+        """mcl_set_lattice_groups: sweep groups of lattices of at most max_bytes one after the other (0: one group;
+        identical results)."""
+        _check(self.lib.mcl_set_lattice_groups(self.h, int(max_bytes)))
+
+    def lattice_groups(self):
+        v = ctypes.c_int32()
+        _check(self.lib.mcl_get_lattice_groups(self.h, ctypes.byref(v)))
+        return v.value
 
     def synchronize(self):
         _check(self.lib.mcl_synchronize(self.h))

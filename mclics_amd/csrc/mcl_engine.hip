@@ -735,30 +735,31 @@ template <int POT, bool MASK, bool SURF, bool FIELD, bool FLIP, bool FREE, bool 
 // its -x side. The workgroup writes kNQS FP64 partials to spart[q * nblk + workgroup].
 // ROW: every lattice of the context is row-aligned (gather_row): site decoding, the y / z neighbour indices and
 // the y / z free faces are scalar work.
+// A launch sweeps the workgroups [b0, b1) of the work list (one lattice group, or all of it); nblk is the length of
+// the whole list (the column stride of spart).
 __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(MEAS && (MASK || POT != MCL_POT_LL) ? 6 : 8, 8))) void k_sweep(
     const LatDev *__restrict__ lats, CtrlDev *__restrict__ ctrl,
                                                 unsigned int *__restrict__ cnt, const int2 *__restrict__ blk,
-                                                int nblk, int chunk, int xcd, int colour,
+                                                int nblk, int b0, int b1, int chunk, int xcd, int colour,
                                                 const unsigned long long *__restrict__ sweep_base, unsigned k,
                                                   float *__restrict__ X, float *__restrict__ Y,
                                                   float *__restrict__ Z, const uint8_t *__restrict__ PT,
                                                   const float *__restrict__ SX, const float *__restrict__ SY,
                                                   const float *__restrict__ SZ, const SurfDev *__restrict__ surf,
-                                                  double *__restrict__ spart, int fin, int nlat,
+                                                  double *__restrict__ spart, int fin, int l0, int nfin,
                                                   const int2 *__restrict__ srange, MomDev *__restrict__ mom,
                                                   ObsDev *__restrict__ obs) {
   const int b = blockIdx.x;
-  // logical workgroups [0, nfin) finalize the measurement folded into the previous sweep (fin = 1), one per lattice:
-  // they come first in dispatch order, so their FP64 eigen-solve runs beside the sweep tiles instead of after them.
-  // nfin is the number of lattices when the context measures in the sweep, else 0.
-  const int nfin = nlat;
+  // logical workgroups [0, nfin) finalize the measurement folded into the previous sweep (fin = 1), one per lattice
+  // l0 + L0: they come first in dispatch order, so their FP64 eigen-solve runs beside the sweep tiles instead of
+  // after them. nfin is the number of lattices of the launch when the context measures in the sweep, else 0.
   const int L0 = xcd ? (b & 7) * chunk + (b >> 3) : b;
   if (L0 < nfin) {
-    if (fin) finalize_compact(L0, spart, nblk, srange[L0], mom, obs);
+    if (fin) finalize_compact(l0 + L0, spart, nblk, srange[l0 + L0], mom, obs);
     return;
   }
-  const int Lb = L0 - nfin;
-  if (Lb >= nblk) return;
+  const int Lb = b0 + L0 - nfin;
+  if (Lb >= b1) return;
   const int2 bd = blk[Lb];
   const int r = bd.x;
   const LatDev &L = lats[r];
@@ -1095,14 +1096,15 @@ __host__ __device__ constexpr bool basic_q(int a) { return a <= 6 || a == 21; }
 // observables, which do, gather them on both colours).
 template <int POT, bool MASK, bool SURF, bool FIELD, bool XQ, bool FREE>
 __global__ __launch_bounds__(kBlock) void k_measure(const LatDev *__restrict__ lats, const CtrlDev *__restrict__ ctrl,
-                                                    const int4 *__restrict__ mblk, int nmblk,
+                                                    const int4 *__restrict__ mblk, int nmblk, int m0,
                                                     const float *__restrict__ X, const float *__restrict__ Y,
                                                     const float *__restrict__ Z, const uint8_t *__restrict__ PT,
                                                     const float *__restrict__ SX, const float *__restrict__ SY,
                                                     const float *__restrict__ SZ, const SurfDev *__restrict__ surf,
                                                     double *__restrict__ partials) {
-  // partials are stored quantity-major, partials[a * nmblk + workgroup], so the finalize reads them coalesced
-  const int Lb = blockIdx.x;
+  // partials are stored quantity-major, partials[a * nmblk + workgroup], so the finalize reads them coalesced; a
+  // launch measures the workgroups m0, m0 + 1, ... of the list (one lattice group, or all of it)
+  const int Lb = m0 + (int)blockIdx.x;
   if (Lb >= nmblk) return;
   const int4 md = mblk[Lb];  // (lattice, colour, first position, last position + 1)
   const int r = md.x, colour = md.y;
@@ -1266,8 +1268,8 @@ __device__ __forceinline__ double eig_sym3_max(const double a[6], double v[3]) {
 __global__ __launch_bounds__(kFinBlock) void k_finalize(const LatDev *__restrict__ lats,
                                                      const double *__restrict__ partials, int nmblk, int xq,
                                                      MomDev *__restrict__ mom, ObsDev *__restrict__ obs,
-                                                     int accumulate, const int2 *__restrict__ ranges) {
-  const int r = blockIdx.x;
+                                                     int accumulate, const int2 *__restrict__ ranges, int l0) {
+  const int r = l0 + (int)blockIdx.x;
   const LatDev &L = lats[r];
   __shared__ double red[kFinWaves][kNQ];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -1474,6 +1476,20 @@ struct LatHost {
   std::vector<float> empty_ni;
 };
 
+// Lattice groups. A context whose lattices together hold more than about the 256 MB Infinity Cache streams every
+// colour pass from HBM: a batch of 64^3 lattices runs at 157 site-updates/ns with 252 MB of directors and at 109 with
+// 302 MB. mcl_sweeps therefore runs such a context group by group, each group a run of consecutive lattices of at most
+// group_bytes, through all the sweeps of the call before the next group starts. Every lattice's trajectory depends
+// only on its own sites and sweep numbers, so the order of the groups changes nothing in the results.
+// A group: lattices [l0, l1), their workgroups [b0, b1) of the sweep work list and [m0, m1) of the measurement list;
+// chunk: the tiles per XCD of an XCD-ordered launch over the group; share: its part of the context's positions.
+struct LatGroup {
+  int l0 = 0, l1 = 0, b0 = 0, b1 = 0, m0 = 0, m1 = 0, chunk = 0;
+  double share = 1.0;
+};
+// Default largest group: directors and point types of both colours.
+constexpr long long kGroupBytes = 192ll << 20;
+
 struct Ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -1500,7 +1516,7 @@ struct Ctx {
   unsigned int *cnt = nullptr;  // striped acceptance counters [lattice][sweep % 3][kCnt]
   unsigned long long sweeps_done = 0;            // host copy of the device sweep base
   unsigned long long *sweep_base = nullptr;      // device sweep base (advanced at the end of every mcl_sweeps)
-  int nblk = 0, chunk = 0;
+  int nblk = 0;  // workgroups of the sweep work list
   int bs = 256, xcd = 1;  // sweep launch shape: threads per workgroup, XCD-contiguous tile order
   int graph_block = 0;    // > 0: sweeps replayed from captured graphs of about this many sweeps
   std::vector<std::pair<std::pair<int, int>, hipGraphExec_t>> graphs;  // ((sweeps, measure_every), graph)
@@ -1522,56 +1538,66 @@ struct Ctx {
   int timing_stride = 0;
   std::vector<hipEvent_t> ev;
   size_t ev_used = 0;
-  std::vector<std::pair<size_t, int>> ev_runs;  // (first event of the pair, kernel launches in the run)
+  // (first event of the pair, colour passes over the whole context in the run: a pass over a lattice group counts
+  // as its share)
+  std::vector<std::pair<size_t, double>> ev_runs;
+  long long group_bytes = kGroupBytes;  // <= 0: one group
+  std::vector<LatGroup> groups;         // the groups mcl_sweeps runs in turn (one when the context fits)
+  LatGroup all;                         // the whole context
 };
 
-using SweepFn = void (*)(dim3, dim3, hipStream_t, const Ctx &, int, unsigned, bool, int);
-using MeasFn = void (*)(dim3, dim3, hipStream_t, const Ctx &);
+using SweepFn = void (*)(dim3, dim3, hipStream_t, const Ctx &, const LatGroup &, int, unsigned, bool, int);
+using MeasFn = void (*)(dim3, dim3, hipStream_t, const Ctx &, int);
 
 template <int POT, bool MASK, bool SURF, bool FIELD, bool FLIP, bool FREE, bool MEAS, bool ROW, bool SPLIT = false>
-void launch_sweep_k(dim3 g, dim3 b, hipStream_t st, const Ctx &c, int colour, unsigned k, int fin) {
+void launch_sweep_k(dim3 g, dim3 b, hipStream_t st, const Ctx &c, const LatGroup &gr, int colour, unsigned k,
+                    int fin) {
   if constexpr (!FLIP && !ROW && !SPLIT) {
     if (c.split) {
-      launch_sweep_k<POT, MASK, SURF, FIELD, FLIP, FREE, MEAS, ROW, true>(g, b, st, c, colour, k, fin);
+      launch_sweep_k<POT, MASK, SURF, FIELD, FLIP, FREE, MEAS, ROW, true>(g, b, st, c, gr, colour, k, fin);
       return;
     }
   }
+  const int nfin = c.fused ? gr.l1 - gr.l0 : 0;
   if constexpr (MASK && POT != MCL_POT_LL) {
     if (!c.has_empty) {
       hipLaunchKernelGGL((k_sweep<POT, MASK, SURF, FIELD, FLIP, FREE, MEAS, ROW, false, SPLIT>), g, b, 0, st,
-                         c.lat_d, c.ctrl_d, c.cnt, c.blk_d, c.nblk, c.chunk, c.xcd, colour, c.sweep_base, k, c.X,
-                         c.Y, c.Z, c.PT, c.SX, c.SY, c.SZ, c.surf_d, c.spart, fin, c.fused ? c.nlat : 0, c.sr_d,
-                         c.mom_d, c.obs_d);
+                         c.lat_d, c.ctrl_d, c.cnt, c.blk_d, c.nblk, gr.b0, gr.b1, gr.chunk, c.xcd, colour,
+                         c.sweep_base, k, c.X, c.Y, c.Z, c.PT, c.SX, c.SY, c.SZ, c.surf_d, c.spart, fin, gr.l0, nfin,
+                         c.sr_d, c.mom_d, c.obs_d);
       return;
     }
   }
   hipLaunchKernelGGL((k_sweep<POT, MASK, SURF, FIELD, FLIP, FREE, MEAS, ROW, true, SPLIT>), g, b, 0, st, c.lat_d,
-                     c.ctrl_d, c.cnt, c.blk_d, c.nblk, c.chunk, c.xcd, colour, c.sweep_base, k, c.X, c.Y, c.Z, c.PT,
-                     c.SX, c.SY, c.SZ, c.surf_d, c.spart, fin, c.fused ? c.nlat : 0, c.sr_d, c.mom_d, c.obs_d);
+                     c.ctrl_d, c.cnt, c.blk_d, c.nblk, gr.b0, gr.b1, gr.chunk, c.xcd, colour, c.sweep_base, k, c.X,
+                     c.Y, c.Z, c.PT, c.SX, c.SY, c.SZ, c.surf_d, c.spart, fin, gr.l0, nfin, c.sr_d, c.mom_d, c.obs_d);
 }
 
 // meas (the measurement folded into the colour-1 pass) is never set for pear contexts: they always measure the pear
 // observables, which keep the separate measurement kernels, so no pear measuring instance is compiled.
 template <int POT, bool MASK, bool SURF, bool FIELD, bool FLIP, bool FREE>
-void launch_sweep(dim3 g, dim3 b, hipStream_t st, const Ctx &c, int colour, unsigned k, bool meas, int fin) {
+void launch_sweep(dim3 g, dim3 b, hipStream_t st, const Ctx &c, const LatGroup &gr, int colour, unsigned k, bool meas,
+                  int fin) {
   if constexpr (POT != MCL_POT_PEAR) {
     if (meas) {
       if (c.row)
-        launch_sweep_k<POT, MASK, SURF, FIELD, FLIP, FREE, true, true>(g, b, st, c, colour, k, fin);
+        launch_sweep_k<POT, MASK, SURF, FIELD, FLIP, FREE, true, true>(g, b, st, c, gr, colour, k, fin);
       else
-        launch_sweep_k<POT, MASK, SURF, FIELD, FLIP, FREE, true, false>(g, b, st, c, colour, k, fin);
+        launch_sweep_k<POT, MASK, SURF, FIELD, FLIP, FREE, true, false>(g, b, st, c, gr, colour, k, fin);
       return;
     }
   }
   if (c.row)
-    launch_sweep_k<POT, MASK, SURF, FIELD, FLIP, FREE, false, true>(g, b, st, c, colour, k, fin);
+    launch_sweep_k<POT, MASK, SURF, FIELD, FLIP, FREE, false, true>(g, b, st, c, gr, colour, k, fin);
   else
-    launch_sweep_k<POT, MASK, SURF, FIELD, FLIP, FREE, false, false>(g, b, st, c, colour, k, fin);
+    launch_sweep_k<POT, MASK, SURF, FIELD, FLIP, FREE, false, false>(g, b, st, c, gr, colour, k, fin);
 }
 
-// Grid of a sweep launch: the work list, plus one workgroup per lattice for the deferred finalize of a measurement
-// folded into the previous sweep.
-dim3 sweep_grid(const Ctx *c) { return dim3(c->xcd ? 8 * c->chunk : c->nblk + (c->fused ? c->nlat : 0)); }
+// Grid of a sweep launch over a lattice group: its part of the work list, plus one workgroup per lattice of the
+// group for the deferred finalize of a measurement folded into the previous sweep.
+dim3 sweep_grid(const Ctx *c, const LatGroup &gr) {
+  return dim3(c->xcd ? 8 * gr.chunk : (gr.b1 - gr.b0) + (c->fused ? gr.l1 - gr.l0 : 0));
+}
 using ResFn = void (*)(hipStream_t, const Ctx &, unsigned, int);
 template <int POT, bool MASK, bool SURF, bool FIELD, bool FLIP, bool FREE, bool ONE>
 void launch_resident_k(hipStream_t st, const Ctx &c, unsigned k0, int nsw) {
@@ -1620,13 +1646,13 @@ ResFn pick_resident(const Ctx &c) {
 }
 
 template <int POT, bool MASK, bool SURF, bool FIELD, bool FREE>
-void launch_measure(dim3 g, dim3 b, hipStream_t st, const Ctx &c) {
+void launch_measure(dim3 g, dim3 b, hipStream_t st, const Ctx &c, int m0) {
   if (c.xq)
     hipLaunchKernelGGL((k_measure<POT, MASK, SURF, FIELD, true, FREE>), g, b, 0, st, c.lat_d, c.ctrl_d, c.mblk_d,
-                       c.nmblk, c.X, c.Y, c.Z, c.PT, c.SX, c.SY, c.SZ, c.surf_d, c.partials);
+                       c.nmblk, m0, c.X, c.Y, c.Z, c.PT, c.SX, c.SY, c.SZ, c.surf_d, c.partials);
   else
     hipLaunchKernelGGL((k_measure<POT, MASK, SURF, FIELD, false, FREE>), g, b, 0, st, c.lat_d, c.ctrl_d, c.mblk_d,
-                       c.nmblk, c.X, c.Y, c.Z, c.PT, c.SX, c.SY, c.SZ, c.surf_d, c.partials);
+                       c.nmblk, m0, c.X, c.Y, c.Z, c.PT, c.SX, c.SY, c.SZ, c.surf_d, c.partials);
 }
 
 template <int POT, bool MASK, bool SURF, bool FIELD, bool FREE>
@@ -1806,9 +1832,47 @@ int build_worklist(Ctx *c, int bs, int xcd) {
     MCL_HIP(hipMemcpy(c->sr_d, sr.data(), sizeof(int2) * sr.size(), hipMemcpyHostToDevice));
   }
   c->nblk = (int)blk.size();
-  c->chunk = (c->nblk + (c->fused ? c->nlat : 0) + 7) / 8;
   c->bs = bs;
   c->xcd = xcd;
+  // lattice groups: consecutive lattices, about equal in bytes, as few as keep each within group_bytes
+  long long all_pos = 0;
+  for (int r = 0; r < c->nlat; r++) all_pos += c->ld[r].nc;
+  auto make_group = [&](int l0, int l1) {
+    LatGroup g;
+    g.l0 = l0;
+    g.l1 = l1;
+    g.b0 = sr[l0].x;
+    g.b1 = sr[l1 - 1].y;
+    g.m0 = c->ld[l0].mp0;
+    g.m1 = c->ld[l1 - 1].mp1;
+    g.chunk = (g.b1 - g.b0 + (c->fused ? l1 - l0 : 0) + 7) / 8;
+    long long pos = 0;
+    for (int r = l0; r < l1; r++) pos += c->ld[r].nc;
+    g.share = (double)pos / (double)all_pos;
+    return g;
+  };
+  c->all = make_group(0, c->nlat);
+  c->groups.clear();
+  std::vector<long long> bytes(c->nlat);
+  long long total = 0;
+  for (int r = 0; r < c->nlat; r++) total += bytes[r] = 2ll * c->ld[r].nc * (12 + (c->mask ? 1 : 0));
+  const long long ng = c->group_bytes > 0 ? std::min<long long>(c->nlat, (total + c->group_bytes - 1) / c->group_bytes) : 1;
+  if (ng <= 1) {
+    c->groups.push_back(c->all);
+  } else {
+    const double target = (double)total / (double)ng;
+    long long cum = 0;
+    int l0 = 0;
+    for (int r = 0; r < c->nlat; r++) {
+      cum += bytes[r];
+      // close the group at the lattice whose middle passes the group's share of the bytes
+      if ((long long)c->groups.size() < ng - 1 && (double)(cum - bytes[r] / 2) >= target * (double)(c->groups.size() + 1)) {
+        c->groups.push_back(make_group(l0, r + 1));
+        l0 = r + 1;
+      }
+    }
+    if (l0 < c->nlat) c->groups.push_back(make_group(l0, c->nlat));
+  }
   for (auto &g : c->graphs) (void)hipGraphExecDestroy(g.second);
   c->graphs.clear();
   return MCL_OK;
@@ -2087,11 +2151,12 @@ int create_ctx(const mcl_lattice_desc *lat, int nlat, Ctx *c) {
 }
 
 
-int run_measure(Ctx *c, int accumulate) {
+// Measurement of the lattices of group gr (c->all: every lattice).
+int run_measure(Ctx *c, int accumulate, const LatGroup &gr) {
   const MeasFn mf = pick_meas(*c);
-  mf(dim3(c->nmblk), dim3(kBlock), c->stream, *c);
-  hipLaunchKernelGGL(k_finalize, dim3(c->nlat), dim3(kFinBlock), 0, c->stream, c->lat_d, c->partials, c->nmblk,
-                     c->xq ? 1 : 0, c->mom_d, c->obs_d, accumulate, (const int2 *)nullptr);
+  mf(dim3(gr.m1 - gr.m0), dim3(kBlock), c->stream, *c, gr.m0);
+  hipLaunchKernelGGL(k_finalize, dim3(gr.l1 - gr.l0), dim3(kFinBlock), 0, c->stream, c->lat_d, c->partials, c->nmblk,
+                     c->xq ? 1 : 0, c->mom_d, c->obs_d, accumulate, (const int2 *)nullptr, gr.l0);
   MCL_HIP(hipGetLastError());
   return MCL_OK;
 }
@@ -2160,9 +2225,9 @@ void mcl_destroy(void *ctx) { destroy_ctx(as_ctx(ctx)); }
 
 // Queues sweeps [k0, k0 + n) relative to the device sweep base (no timing, no advance).
 // The finalize of a measurement taken by the colour-1 sweep pass.
-static int run_fused_finalize(Ctx *c) {
-  hipLaunchKernelGGL(k_finalize, dim3(c->nlat), dim3(kFinBlock), 0, c->stream, c->lat_d, c->spart, c->nblk, 0,
-                     c->mom_d, c->obs_d, 1, (const int2 *)c->sr_d);
+static int run_fused_finalize(Ctx *c, const LatGroup &gr) {
+  hipLaunchKernelGGL(k_finalize, dim3(gr.l1 - gr.l0), dim3(kFinBlock), 0, c->stream, c->lat_d, c->spart, c->nblk, 0,
+                     c->mom_d, c->obs_d, 1, (const int2 *)c->sr_d, gr.l0);
   MCL_HIP(hipGetLastError());
   return MCL_OK;
 }
@@ -2170,33 +2235,39 @@ static int run_fused_finalize(Ctx *c) {
 // One sweep (both colour passes) and, when measure is set, its measurement: folded into the colour-1 pass when the
 // context allows it (its finalize then runs in the extra workgroups of the next colour-0 pass, *pending, or in a
 // k_finalize launch at the end of the call), else the measurement kernels after the pass.
-static int sweep_once(Ctx *c, SweepFn sf, dim3 grid, dim3 block, unsigned k, bool measure, bool *pending) {
-  sf(grid, block, c->stream, *c, 0, k, false, *pending ? 1 : 0);
+static int sweep_once(Ctx *c, SweepFn sf, const LatGroup &gr, dim3 grid, dim3 block, unsigned k, bool measure,
+                      bool *pending) {
+  sf(grid, block, c->stream, *c, gr, 0, k, false, *pending ? 1 : 0);
   *pending = false;
-  sf(grid, block, c->stream, *c, 1, k, measure && c->fused, 0);
+  sf(grid, block, c->stream, *c, gr, 1, k, measure && c->fused, 0);
   if (!measure) return MCL_OK;
   if (c->fused) {
     *pending = true;
     return MCL_OK;
   }
-  return run_measure(c, 1);
+  return run_measure(c, 1, gr);
 }
 
-static int flush_finalize(Ctx *c, bool *pending) {
+static int flush_finalize(Ctx *c, const LatGroup &gr, bool *pending) {
   if (!*pending) return MCL_OK;
   *pending = false;
-  return run_fused_finalize(c);
+  return run_fused_finalize(c, gr);
 }
 
+// The sweeps [k0, k0 + n), lattice group after lattice group.
 static int enqueue_sweeps(Ctx *c, SweepFn sf, int k0, int n, int measure_every) {
-  const dim3 grid = sweep_grid(c), block(c->bs);
-  bool pending = false;
-  for (int s = 0; s < n; s++) {
-    const int st = sweep_once(c, sf, grid, block, (unsigned)(k0 + s),
-                              measure_every > 0 && (s + 1) % measure_every == 0, &pending);
-    if (st) return st;
+  const dim3 block(c->bs);
+  for (const LatGroup &gr : c->groups) {
+    const dim3 grid = sweep_grid(c, gr);
+    bool pending = false;
+    for (int s = 0; s < n; s++) {
+      const int st = sweep_once(c, sf, gr, grid, block, (unsigned)(k0 + s),
+                                measure_every > 0 && (s + 1) % measure_every == 0, &pending);
+      if (st) return st;
+    }
+    if (int st = flush_finalize(c, gr, &pending)) return st;
   }
-  return flush_finalize(c, &pending);
+  return MCL_OK;
 }
 
 // Captured graphs a context keeps; the least recently replayed one is dropped beyond this.
@@ -2268,7 +2339,7 @@ static int replay(Ctx *c, hipGraphExec_t g, int len) {
   MCL_HIP(hipGraphLaunch(g, c->stream));
   if (timed) {
     MCL_HIP(hipEventRecord(c->ev[c->ev_used + 1], c->stream));
-    c->ev_runs.push_back({c->ev_used, 2 * len});
+    c->ev_runs.push_back({c->ev_used, 2.0 * len});
     c->ev_used += 2;
   }
   c->sweeps_done += (unsigned long long)len;
@@ -2305,7 +2376,7 @@ int mcl_sweeps(void *ctx, int32_t nsweeps, int32_t measure_every) {
     return MCL_OK;
   }
   const SweepFn sf = pick_sweep(*c);
-  const dim3 grid = sweep_grid(c), block(c->bs);
+  const dim3 block(c->bs);
   int done = 0;
   const GraphPlan plan = graph_plan(c, nsweeps, measure_every);
   if (plan.nblocks > 0) {
@@ -2322,39 +2393,42 @@ int mcl_sweeps(void *ctx, int32_t nsweeps, int32_t measure_every) {
   }
   const int rest = nsweeps - done;
   if (rest > 0) {
-    // Kernel timing: one event pair around each run of consecutive sweep launches (up to timing_stride sweeps,
-    // cut at every measurement), so the markers do not sit between the timed kernels.
-    int run_len = 0;
-    size_t run_start = 0;
-    bool pending = false;
-    for (int s = 0; s < rest; s++) {
-      if (c->timing_stride > 0 && run_len == 0 && c->ev_used + 2 <= c->ev.size()) {
-        run_start = c->ev_used;
-        c->ev_used += 2;
-        MCL_HIP(hipEventRecord(c->ev[run_start], c->stream));
-        run_len = 1;
-      } else if (run_len > 0) {
-        run_len++;
-      }
-      const bool measure_now = measure_every > 0 && (s + 1) % measure_every == 0;
-      sf(grid, block, c->stream, *c, 0, (unsigned)s, false, pending ? 1 : 0);
-      pending = false;
-      sf(grid, block, c->stream, *c, 1, (unsigned)s, measure_now && c->fused, 0);
-      if (run_len > 0 && (measure_now || run_len == c->timing_stride || s == rest - 1)) {
-        MCL_HIP(hipEventRecord(c->ev[run_start + 1], c->stream));
-        c->ev_runs.push_back({run_start, 2 * run_len});
-        run_len = 0;
-      }
-      if (measure_now) {
-        if (c->fused) {
-          pending = true;
-        } else {
-          const int st = run_measure(c, 1);
-          if (st) return st;
+    // Kernel timing: one event pair around each run of consecutive sweep launches of a lattice group (up to
+    // timing_stride sweeps, cut at every measurement), so the markers do not sit between the timed kernels.
+    for (const LatGroup &gr : c->groups) {
+      const dim3 grid = sweep_grid(c, gr);
+      int run_len = 0;
+      size_t run_start = 0;
+      bool pending = false;
+      for (int s = 0; s < rest; s++) {
+        if (c->timing_stride > 0 && run_len == 0 && c->ev_used + 2 <= c->ev.size()) {
+          run_start = c->ev_used;
+          c->ev_used += 2;
+          MCL_HIP(hipEventRecord(c->ev[run_start], c->stream));
+          run_len = 1;
+        } else if (run_len > 0) {
+          run_len++;
+        }
+        const bool measure_now = measure_every > 0 && (s + 1) % measure_every == 0;
+        sf(grid, block, c->stream, *c, gr, 0, (unsigned)s, false, pending ? 1 : 0);
+        pending = false;
+        sf(grid, block, c->stream, *c, gr, 1, (unsigned)s, measure_now && c->fused, 0);
+        if (run_len > 0 && (measure_now || run_len == c->timing_stride || s == rest - 1)) {
+          MCL_HIP(hipEventRecord(c->ev[run_start + 1], c->stream));
+          c->ev_runs.push_back({run_start, 2.0 * run_len * gr.share});
+          run_len = 0;
+        }
+        if (measure_now) {
+          if (c->fused) {
+            pending = true;
+          } else {
+            const int st = run_measure(c, 1, gr);
+            if (st) return st;
+          }
         }
       }
+      if (int st = flush_finalize(c, gr, &pending)) return st;
     }
-    if (int st = flush_finalize(c, &pending)) return st;
     hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, c->stream, c->sweep_base, (unsigned)rest);
     c->sweeps_done += (unsigned long long)rest;
   }
@@ -2382,15 +2456,14 @@ int mcl_kernel_timing(void *ctx, int64_t *launches, double *total_ms) {
   Ctx *c = as_ctx(ctx);
   if (!c || !launches || !total_ms) return fail(MCL_ERR_ARG, "null argument");
   MCL_HIP(hipStreamSynchronize(c->stream));
-  double tot = 0.0;
-  int64_t n = 0;
+  double tot = 0.0, n = 0.0;
   for (auto &p : c->ev_runs) {
     float ms = 0.0f;
     MCL_HIP(hipEventElapsedTime(&ms, c->ev[p.first], c->ev[p.first + 1]));
     tot += ms;
     n += p.second;
   }
-  *launches = n;
+  *launches = (int64_t)std::llround(n);
   *total_ms = tot;
   return MCL_OK;
 }
@@ -2504,7 +2577,7 @@ int mcl_measure(void *ctx, int32_t lat, mcl_observables *out) {
   int st = check_lat(c, lat);
   if (st) return st;
   if (!out) return fail(MCL_ERR_ARG, "null argument");
-  if ((st = run_measure(c, 0))) return st;
+  if ((st = run_measure(c, 0, c->all))) return st;
   ObsDev o;
   MCL_HIP(hipMemcpyAsync(&o, c->obs_d + lat, sizeof(o), hipMemcpyDeviceToHost, c->stream));
   MCL_HIP(hipStreamSynchronize(c->stream));
@@ -2736,6 +2809,20 @@ int mcl_set_graph(void *ctx, int32_t block_sweeps) {
   return MCL_OK;
 }
 
+int mcl_set_lattice_groups(void *ctx, int64_t max_bytes) {
+  Ctx *c = as_ctx(ctx);
+  if (!c) return fail(MCL_ERR_ARG, "null context");
+  c->group_bytes = max_bytes > 0 ? (long long)max_bytes : 0;
+  return build_worklist(c, c->bs, c->xcd);
+}
+
+int mcl_get_lattice_groups(void *ctx, int32_t *ngroups) {
+  Ctx *c = as_ctx(ctx);
+  if (!c || !ngroups) return fail(MCL_ERR_ARG, "null argument");
+  *ngroups = (int32_t)c->groups.size();
+  return MCL_OK;
+}
+
 int mcl_set_resident(void *ctx, int32_t enable) {
   Ctx *c = as_ctx(ctx);
   if (!c) return fail(MCL_ERR_ARG, "null context");
@@ -2767,13 +2854,14 @@ std::string tune_key(const Ctx *c) {
   int lp = 0, lm = 0;
   while ((1ll << lp) < c->pool) lp++;
   while ((1 << lm) < maxnc) lm++;
-  char buf[160];
+  char buf[192];
   // every flag that selects the kernel instance (so contexts that run different kernels do not share a result),
   // then the sizes
   snprintf(buf, sizeof(buf),
-           "sweep:pot%d:mask%d:surf%d:field%d:flip%d:free%d:row%d:split%d:fused%d:empty%d:xq%d:nlat%d:pool2^%d:maxnc2^%d",
+           "sweep:pot%d:mask%d:surf%d:field%d:flip%d:free%d:row%d:split%d:fused%d:empty%d:xq%d:nlat%d:pool2^%d:maxnc2^%d"
+           ":groupmb%lld",
            c->pot, c->mask, c->surf, c->field, c->flip, c->free, c->row, c->split, c->fused, c->has_empty, c->xq,
-           c->nlat, lp, lm);
+           c->nlat, lp, lm, c->group_bytes > 0 ? c->group_bytes >> 20 : 0ll);
   return buf;
 }
 
