@@ -1089,6 +1089,11 @@ __global__ __launch_bounds__(kResBlock) void k_sweep_resident(
 
 // The basic quantities (energy, Q sums, count) are 0..6 and 21; XQ adds the radial, polar and pear ones.
 __host__ __device__ constexpr bool basic_q(int a) { return a <= 6 || a == 21; }
+// The quantities a measurement of potential POT writes: the pear observables (13..20) only for pear contexts.
+template <int POT, bool XQ>
+__host__ __device__ constexpr bool written_q(int a) {
+  return (XQ || basic_q(a)) && (POT == MCL_POT_PEAR || a < 13 || a > 20);
+}
 
 // The energy of a lattice counts every bond twice (once from each end, as the reference's per-site sum does) plus
 // the one-site terms of every site. Every bond has exactly one colour-0 end, so the colour-0 workgroups add twice
@@ -1203,15 +1208,18 @@ __global__ __launch_bounds__(kBlock) void k_measure(const LatDev *__restrict__ l
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
   for (int a = 0; a < kNQ; a++) {
-    if (!XQ && !basic_q(a)) continue;
+    if (!written_q<POT, XQ>(a)) continue;
     const double v = wave_sum(q[a]);
     if (lane == 0) red[w][a] = v;
   }
   __syncthreads();
+  // every quantity k_finalize reads gets a partial; those this potential never writes are zero
   if (threadIdx.x < kNQ && (XQ || basic_q(threadIdx.x))) {
     double v = 0.0;
+    if (written_q<POT, XQ>(threadIdx.x)) {
 #pragma unroll
-    for (int ww = 0; ww < kWaves; ww++) v += red[ww][threadIdx.x];
+      for (int ww = 0; ww < kWaves; ww++) v += red[ww][threadIdx.x];
+    }
     partials[(size_t)threadIdx.x * nmblk + Lb] = v;
   }
 }
