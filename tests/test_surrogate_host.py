@@ -130,7 +130,10 @@ def test_training_distribution():
     f = np.array([s["field"] for s in specs])
     sph = np.array([s["geometry"] == "sphere" for s in specs])
     heat = np.array([s["heating"] for s in specs])
-    assert set(L % 2) == {0} and L.min() == 16 and L.max() == 64
+    # small lattices by default (the network places points and warm-starts; sizes beyond are asked at the largest)
+    assert set(L % 2) == {0} and L.min() == 12 and L.max() == 20
+    L2 = np.array([s["L"] for s in surrogate.random_lattices(500, rng, lmin=16, lmax=64)])
+    assert set(L2 % 2) == {0} and L2.min() == 16 and L2.max() == 64
     assert A.min() >= 0.8 and A.max() <= 1.2
     assert np.all(W[~sph] == 0) and W[sph].min() >= 0.5 and W[sph].max() <= 1.5
     assert 0.4 < (f == 0).mean() < 0.6 and f.max() <= 0.3
