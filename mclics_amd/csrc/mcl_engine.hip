@@ -1281,7 +1281,8 @@ __device__ __forceinline__ double eig_sym3_max(const double a[6], double v[3]) {
 __global__ __launch_bounds__(kFinBlock) void k_finalize(const LatDev *__restrict__ lats,
                                                      const double *__restrict__ partials, int nmblk, int xq,
                                                      MomDev *__restrict__ mom, ObsDev *__restrict__ obs,
-                                                     int accumulate, const int2 *__restrict__ ranges, int l0) {
+                                                     int accumulate, const int2 *__restrict__ ranges, int l0,
+                                                     unsigned long long *__restrict__ sweep_base, unsigned advance) {
   const int r = l0 + (int)blockIdx.x;
   const LatDev &L = lats[r];
   __shared__ double red[kFinWaves][kNQ];
@@ -1290,11 +1291,24 @@ __global__ __launch_bounds__(kFinBlock) void k_finalize(const LatDev *__restrict
 #pragma unroll
   for (int a = 0; a < kNQ; a++) v[a] = 0.0;
   if (ranges) {
+    // four partials per lane in flight, added in increasing order (the order finalize_compact keeps)
     const int2 rg = ranges[r];
-    for (int p = rg.x + (int)threadIdx.x; p < rg.y; p += kFinBlock) {
+#pragma unroll 1
+    for (int p = rg.x + (int)threadIdx.x; p < rg.y; p += 4 * kFinBlock) {
+      double x[4][kNQS];
 #pragma unroll
-      for (int a = 0; a < 7; a++) v[a] += partials[(size_t)a * nmblk + p];
-      v[21] += partials[(size_t)7 * nmblk + p];
+      for (int u = 0; u < 4; u++) {
+        const int q = p + u * kFinBlock;
+#pragma unroll
+        for (int a = 0; a < kNQS; a++) x[u][a] = q < rg.y ? partials[(size_t)a * nmblk + q] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        if (p + u * kFinBlock >= rg.y) continue;
+#pragma unroll
+        for (int a = 0; a < 7; a++) v[a] += x[u][a];
+        v[21] += x[u][7];
+      }
     }
   } else {
     for (int p = L.mp0 + (int)threadIdx.x; p < L.mp1; p += kFinBlock) {
@@ -1358,6 +1372,8 @@ __global__ __launch_bounds__(kFinBlock) void k_finalize(const LatDev *__restrict
     m.divn += o.divn;
     mom[r] = m;
   }
+  // the last finalize of a call advances the sweep base itself (instead of a k_advance launch after it)
+  if (advance && blockIdx.x == 0) *sweep_base += advance;
 }
 
 __global__ void k_advance(unsigned long long *sweep_base, unsigned n) {
@@ -2176,12 +2192,14 @@ int create_ctx(const mcl_lattice_desc *lat, int nlat, Ctx *c) {
 }
 
 
-// Measurement of the lattices of group gr (c->all: every lattice).
-int run_measure(Ctx *c, int accumulate, const LatGroup &gr) {
+// Measurement of the lattices of group gr (c->all: every lattice); advance > 0: its finalize also advances the device
+// sweep base by that much.
+int run_measure(Ctx *c, int accumulate, const LatGroup &gr, unsigned advance = 0) {
   const MeasFn mf = pick_meas(*c);
   mf(dim3(gr.m1 - gr.m0), dim3(kBlock), c->stream, *c, gr.m0);
   hipLaunchKernelGGL(k_finalize, dim3(gr.l1 - gr.l0), dim3(kFinBlock), 0, c->stream, c->lat_d, c->partials, c->nmblk,
-                     c->xq ? 1 : 0, c->mom_d, c->obs_d, accumulate, (const int2 *)nullptr, gr.l0);
+                     c->xq ? 1 : 0, c->mom_d, c->obs_d, accumulate, (const int2 *)nullptr, gr.l0, c->sweep_base,
+                     advance);
   MCL_HIP(hipGetLastError());
   return MCL_OK;
 }
@@ -2250,9 +2268,9 @@ void mcl_destroy(void *ctx) { destroy_ctx(as_ctx(ctx)); }
 
 // Queues sweeps [k0, k0 + n) relative to the device sweep base (no timing, no advance).
 // The finalize of a measurement taken by the colour-1 sweep pass.
-static int run_fused_finalize(Ctx *c, const LatGroup &gr) {
+static int run_fused_finalize(Ctx *c, const LatGroup &gr, unsigned advance) {
   hipLaunchKernelGGL(k_finalize, dim3(gr.l1 - gr.l0), dim3(kFinBlock), 0, c->stream, c->lat_d, c->spart, c->nblk, 0,
-                     c->mom_d, c->obs_d, 1, (const int2 *)c->sr_d, gr.l0);
+                     c->mom_d, c->obs_d, 1, (const int2 *)c->sr_d, gr.l0, c->sweep_base, advance);
   MCL_HIP(hipGetLastError());
   return MCL_OK;
 }
@@ -2260,8 +2278,10 @@ static int run_fused_finalize(Ctx *c, const LatGroup &gr) {
 // One sweep (both colour passes) and, when measure is set, its measurement: folded into the colour-1 pass when the
 // context allows it (its finalize then runs in the extra workgroups of the next colour-0 pass, *pending, or in a
 // k_finalize launch at the end of the call), else the measurement kernels after the pass.
+// *advance (when not 0) is what the device sweep base still has to gain at the end of the queued work: the
+// measurement of the call's last sweep takes it into its finalize and clears it, so no k_advance launch follows.
 static int sweep_once(Ctx *c, SweepFn sf, const LatGroup &gr, dim3 grid, dim3 block, unsigned k, bool measure,
-                      bool *pending) {
+                      bool *pending, bool last, unsigned *advance) {
   sf(grid, block, c->stream, *c, gr, 0, k, false, *pending ? 1 : 0);
   *pending = false;
   sf(grid, block, c->stream, *c, gr, 1, k, measure && c->fused, 0);
@@ -2270,29 +2290,42 @@ static int sweep_once(Ctx *c, SweepFn sf, const LatGroup &gr, dim3 grid, dim3 bl
     *pending = true;
     return MCL_OK;
   }
-  return run_measure(c, 1, gr);
+  const unsigned adv = last ? *advance : 0u;
+  if (last) *advance = 0;
+  return run_measure(c, 1, gr, adv);
 }
 
-static int flush_finalize(Ctx *c, const LatGroup &gr, bool *pending) {
+static int flush_finalize(Ctx *c, const LatGroup &gr, bool *pending, bool last, unsigned *advance) {
   if (!*pending) return MCL_OK;
   *pending = false;
-  return run_fused_finalize(c, gr);
+  const unsigned adv = last ? *advance : 0u;
+  if (last) *advance = 0;
+  return run_fused_finalize(c, gr, adv);
 }
 
-// The sweeps [k0, k0 + n), lattice group after lattice group.
-static int enqueue_sweeps(Ctx *c, SweepFn sf, int k0, int n, int measure_every) {
+// Advances the device sweep base unless the call's last finalize already did.
+static int finish_advance(Ctx *c, unsigned advance) {
+  if (advance) hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, c->stream, c->sweep_base, advance);
+  return MCL_OK;
+}
+
+// The sweeps [k0, k0 + n), lattice group after lattice group, then the advance of the sweep base by `advance`.
+static int enqueue_sweeps(Ctx *c, SweepFn sf, int k0, int n, int measure_every, unsigned advance) {
   const dim3 block(c->bs);
-  for (const LatGroup &gr : c->groups) {
+  for (size_t g = 0; g < c->groups.size(); g++) {
+    const LatGroup &gr = c->groups[g];
+    const bool last_group = g + 1 == c->groups.size();
     const dim3 grid = sweep_grid(c, gr);
     bool pending = false;
     for (int s = 0; s < n; s++) {
       const int st = sweep_once(c, sf, gr, grid, block, (unsigned)(k0 + s),
-                                measure_every > 0 && (s + 1) % measure_every == 0, &pending);
+                                measure_every > 0 && (s + 1) % measure_every == 0, &pending,
+                                last_group && s == n - 1, &advance);
       if (st) return st;
     }
-    if (int st = flush_finalize(c, gr, &pending)) return st;
+    if (int st = flush_finalize(c, gr, &pending, last_group, &advance)) return st;
   }
-  return MCL_OK;
+  return finish_advance(c, advance);
 }
 
 // Captured graphs a context keeps; the least recently replayed one is dropped beyond this.
@@ -2315,8 +2348,7 @@ static int get_graph(Ctx *c, SweepFn sf, int len, int measure_every, hipGraphExe
   }
   hipGraph_t graph = nullptr;
   MCL_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-  int st = enqueue_sweeps(c, sf, 0, len, measure_every);
-  hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, c->stream, c->sweep_base, (unsigned)len);
+  int st = enqueue_sweeps(c, sf, 0, len, measure_every, (unsigned)len);
   const hipError_t e = hipStreamEndCapture(c->stream, &graph);
   if (st) {
     if (graph) (void)hipGraphDestroy(graph);
@@ -2420,7 +2452,10 @@ int mcl_sweeps(void *ctx, int32_t nsweeps, int32_t measure_every) {
   if (rest > 0) {
     // Kernel timing: one event pair around each run of consecutive sweep launches of a lattice group (up to
     // timing_stride sweeps, cut at every measurement), so the markers do not sit between the timed kernels.
-    for (const LatGroup &gr : c->groups) {
+    unsigned advance = (unsigned)rest;
+    for (size_t g = 0; g < c->groups.size(); g++) {
+      const LatGroup &gr = c->groups[g];
+      const bool last_group = g + 1 == c->groups.size();
       const dim3 grid = sweep_grid(c, gr);
       int run_len = 0;
       size_t run_start = 0;
@@ -2447,14 +2482,16 @@ int mcl_sweeps(void *ctx, int32_t nsweeps, int32_t measure_every) {
           if (c->fused) {
             pending = true;
           } else {
-            const int st = run_measure(c, 1, gr);
+            const bool last = last_group && s == rest - 1;
+            const int st = run_measure(c, 1, gr, last ? advance : 0u);
+            if (last) advance = 0;
             if (st) return st;
           }
         }
       }
-      if (int st = flush_finalize(c, gr, &pending)) return st;
+      if (int st = flush_finalize(c, gr, &pending, last_group, &advance)) return st;
     }
-    hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, c->stream, c->sweep_base, (unsigned)rest);
+    if (int st = finish_advance(c, advance)) return st;
     c->sweeps_done += (unsigned long long)rest;
   }
   MCL_HIP(hipGetLastError());
