@@ -6,10 +6,10 @@ homogeneous along z, fixed T = 1.10 (just below T_NI ~ 1.123), Metropolis rotati
 maximum angle (ang_var rule, about 50 % acceptance), FP64 lattice reductions for S and E every 10 sweeps.
 A step is one sweep: one Metropolis trial per site of the lattice. The default run times 20000 sweeps, the
 length of the benchmark configuration, after 2000 warmup sweeps; the directors stay resident in HBM throughout.
-The sweeps run as replays of captured HIP graphs (the library's production path: 50-sweep blocks, and one graph of
-the remaining length for a call that is not a multiple of 50, such as a 20-sweep run), with the launch shape from the
-library's autotuner; the graphs are captured before the warmup. The measurement every 10 sweeps is folded into the
-sweep kernel itself.
+The sweeps are launched eagerly (two kernel launches per sweep), with the launch shape from the library's
+autotuner; one pair of HIP events brackets the timed call on the library's stream. (--graph 50 replays captured
+50-sweep HIP graphs instead, the mclics program's default, with one event pair per replay: the same rate over 20000
+sweeps, 1-2 % slower for a 20-sweep call.) The measurement every 10 sweeps is folded into the sweep kernel itself.
 
     python bench.py [--gpus 1] [--steps 20000] [--warmup 2000] [--no-cpu-baseline]
 
@@ -378,7 +378,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20000)
     ap.add_argument("--warmup", type=int, default=2000)
-    ap.add_argument("--timing-stride", type=int, default=10, help="time the kernels of every n-th sweep")
+    ap.add_argument("--timing-stride", type=int, default=None,
+                    help="sweeps per event pair of the kernel timing (default: the whole timed call for the 128^3 "
+                         "run, 10 for the slab and batch runs)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-bytes", type=int, default=256 << 20,
                     help="size of the STREAM-style copy that checks the HBM peak (0: skip)")
@@ -386,10 +388,14 @@ def main():
     ap.add_argument("--launch", default=None, metavar="THREADS,XCD_ORDER",
                     help="fixed launch shape instead of the autotuner's choice (for profiling runs, whose tracer "
                          "changes the timings the autotuner compares)")
-    ap.add_argument("--graph", type=int, default=50,
-                    help="sweeps per captured HIP graph in the 128^3 run (0: eager launches, events every "
-                         "--timing-stride sweeps)")
+    ap.add_argument("--graph", type=int, default=None,
+                    help="sweeps per captured HIP graph (default: 0, eager launches, for the 128^3 run; 50 for the "
+                         "smalllat run)")
     a = ap.parse_args()
+    if a.graph is None:
+        a.graph = 50 if a.workload == "smalllat" else 0
+    if a.timing_stride is None:
+        a.timing_stride = max(1, a.steps) if a.workload == "ll128" else 10
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if a.gpus != 1 or world != 1:
         print("bench.py runs on one GPU only: the benchmark lattice is one 128^3 system on one MI355X "

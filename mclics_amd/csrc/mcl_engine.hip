@@ -1152,7 +1152,8 @@ __global__ __launch_bounds__(kBlock) void k_measure(const LatDev *__restrict__ l
       if (L.flags & F_RSPH) {
         const float rr = sqrtf(rx * rx + ry * ry + rz * rz);
         if (rr >= 0.5f) {
-          const double c = (double)(n.x * rx + n.y * ry + n.z * rz) / rr;
+          // c = n.r / r in FP32 (a float quotient widened to double), as parameter_order.cpp:236 computes it
+          const double c = (double)((n.x * rx + n.y * ry + n.z * rz) / rr);
           q[7] += 1.5 * c * c - 0.5;
           q[8] += c;
           q[9] += 1.0;
@@ -1161,7 +1162,7 @@ __global__ __launch_bounds__(kBlock) void k_measure(const LatDev *__restrict__ l
       if (L.flags & F_RCYL) {
         const float rr = sqrtf(rx * rx + ry * ry);
         if (rr >= 0.5f) {
-          const double c = (double)(n.x * rx + n.y * ry) / rr;
+          const double c = (double)((n.x * rx + n.y * ry) / rr);
           q[10] += 1.5 * c * c - 0.5;
           q[11] += c;
           q[12] += 1.0;
@@ -2451,7 +2452,9 @@ int mcl_sweeps(void *ctx, int32_t nsweeps, int32_t measure_every) {
   const int rest = nsweeps - done;
   if (rest > 0) {
     // Kernel timing: one event pair around each run of consecutive sweep launches of a lattice group (up to
-    // timing_stride sweeps, cut at every measurement), so the markers do not sit between the timed kernels.
+    // timing_stride sweeps, cut where separate measurement kernels follow a sweep), so that the markers do not sit
+    // between the timed kernels and the measurement kernels are not counted as sweep time. A measurement folded into
+    // the sweep adds no kernel, so it does not cut the run.
     unsigned advance = (unsigned)rest;
     for (size_t g = 0; g < c->groups.size(); g++) {
       const LatGroup &gr = c->groups[g];
@@ -2473,7 +2476,7 @@ int mcl_sweeps(void *ctx, int32_t nsweeps, int32_t measure_every) {
         sf(grid, block, c->stream, *c, gr, 0, (unsigned)s, false, pending ? 1 : 0);
         pending = false;
         sf(grid, block, c->stream, *c, gr, 1, (unsigned)s, measure_now && c->fused, 0);
-        if (run_len > 0 && (measure_now || run_len == c->timing_stride || s == rest - 1)) {
+        if (run_len > 0 && ((measure_now && !c->fused) || run_len == c->timing_stride || s == rest - 1)) {
           MCL_HIP(hipEventRecord(c->ev[run_start + 1], c->stream));
           c->ev_runs.push_back({run_start, 2.0 * run_len * gr.share});
           run_len = 0;
