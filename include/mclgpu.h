@@ -139,6 +139,12 @@ int mcl_reset_moments(void *ctx);
 int mcl_get_moments(void *ctx, int32_t lat, mcl_moments *out);
 /* Measures the current state now without accumulating it (synchronizes). */
 int mcl_measure(void *ctx, int32_t lat, mcl_observables *out);
+/* *active = 1 when the measurements of mcl_sweeps are folded into the colour-1 sweep pass (no reference
+ * counterpart: the reference measures after the sweep, evolve_thermal.cu:87-92). Every context but those measuring
+ * the pear observables, GHRL and pear contexts with empty sites, lattices one site wide in x, and lattices measuring
+ * both radial orders; the others measure with separate kernels after the sweep. The moments agree either way (E to
+ * FP32 rounding, S and the radial order to FP64 rounding). */
+int mcl_measure_in_sweep(void *ctx, int32_t *active);
 
 /* Director transfer in the AoS host layout (the cudaMemcpy of d_ni, evolve_thermal.cu:90 and :99). */
 int mcl_get_directors(void *ctx, int32_t lat, float *ni);

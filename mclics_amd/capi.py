@@ -105,6 +105,7 @@ ABI = {
     "mcl_prepare_sweeps": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]),
     "mcl_set_resident": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "mcl_resident": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32)]),
+    "mcl_measure_in_sweep": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32)]),
     "mcl_set_lattice_groups": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64]),
     "mcl_get_lattice_groups": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32)]),
     "mcl_mlp_create": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_uint64,
@@ -347,6 +348,11 @@ class Engine:
     def resident(self):
         v = ctypes.c_int32()
         _check(self.lib.mcl_resident(self.h, ctypes.byref(v)))
+        return bool(v.value)
+
+    def measure_in_sweep(self):
+        v = ctypes.c_int32()
+        _check(self.lib.mcl_measure_in_sweep(self.h, ctypes.byref(v)))
         return bool(v.value)
 
     def set_lattice_groups(self, max_bytes):

@@ -594,14 +594,20 @@ __host__ __device__ __forceinline__ float ang_var_rule(float av, unsigned acc, i
 __device__ __forceinline__ double eig_sym3_max(const double a[6], double v[3]);
 
 // Compact layout of the measurement partials written by the sweep itself: 0 E, 1..5 sum n_a n_b (xx xy xz yy yz),
-// 6 count, 7 sum n_z^2.
+// 6 count, 7 sum n_z^2; in contexts that measure a radial order (sphere or cylinder, one per lattice), also
+// 8 sum (3 c^2 - 1) / 2, 9 sum c, 10 count of the sites off the centre (c = n.r / r, parameter_order.cpp:223-245).
 constexpr int kNQS = 8;
+constexpr int kNQSR = 11;
+__host__ __device__ constexpr bool radial_flags(int flags) { return (flags & (F_RSPH | F_RCYL)) != 0; }
 
 // Finalize of one lattice's compact partials [rg.x, rg.y) (column stride `stride`) by a whole workgroup of any
-// multiple of 64 threads: fixed-order sum, observables, and the sample added to the moments. The quantities of the
-// radial and pear observables are zero in this layout.
-__device__ __forceinline__ void finalize_compact(int r, const double *__restrict__ part, int stride, int2 rg,
-                                 MomDev *__restrict__ mom, ObsDev *__restrict__ obs) {
+// multiple of 64 threads: fixed-order sum, observables, and the sample added to the moments. A lattice with a radial
+// order (flags) has kNQSR quantities, its radial sums in rows 8-10; the pear observables are not in this layout.
+// RAD: the lattice may carry a radial order (the generic sweep instances; a context measuring one never takes the
+// row-aligned path, whose plain passes keep this finalize at its smallest).
+template <bool RAD>
+__device__ __forceinline__ void finalize_compact(int r, int flags, const double *__restrict__ part, int stride,
+                                                 int2 rg, MomDev *__restrict__ mom, ObsDev *__restrict__ obs) {
   // The summation order of k_finalize whatever the sweep's block size: kFinWaves virtual waves of 64 lanes, lane l of
   // virtual wave vw adding partials rg.x + 64 vw + l, + kFinBlock, ...; each virtual wave summed by wave_sum; the
   // wave sums added in order. A measured sweep then gets the same moments, bit for bit, whether its finalize runs
@@ -610,11 +616,12 @@ __device__ __forceinline__ void finalize_compact(int r, const double *__restrict
   // inlined into every sweep instance, whose plain passes are held to 64 VGPRs. A lane loads four of its partials
   // at a time and adds them in increasing order, so the one finalizing workgroup waits for few memory round trips
   // and ends well within the colour pass it runs beside.
-  __shared__ double fred[kFinWaves][kNQS];
+  __shared__ double fred[kFinWaves][RAD ? kNQSR : kNQS];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = (int)(blockDim.x >> 6);
+  const int nq = RAD && radial_flags(flags) ? kNQSR : kNQS;
 #pragma unroll 1
-  for (int j = w; j < kFinWaves * kNQS; j += nw) {
-    const int vw = j / kNQS, a = j - vw * kNQS;
+  for (int j = w; j < kFinWaves * nq; j += nw) {
+    const int vw = !RAD || nq == kNQS ? j / kNQS : j / kNQSR, a = j - vw * nq;
     const double *__restrict__ col = part + (size_t)a * stride;
     double v = 0.0;
 #pragma unroll 1
@@ -652,6 +659,24 @@ __device__ __forceinline__ void finalize_compact(int r, const double *__restrict
   double qm[6] = {o.Q[0], o.Q[1], o.Q[2], o.Q[3], o.Q[4], -o.Q[0] - o.Q[3]};
   o.S = N > 0.0 ? eig_sym3_max(qm, o.dir) : 0.0;
   if (N <= 0.0) { o.dir[0] = 0; o.dir[1] = 0; o.dir[2] = 1; }
+  if (RAD && nq > kNQS) {
+    // the radial order, in k_finalize's order and arithmetic (its tot[7..9] or tot[10..12])
+    double rs = 0.0, rp = 0.0, rn = 0.0;
+#pragma unroll 1
+    for (int ww = 0; ww < kFinWaves; ww++) {
+      rs += fred[ww][8];
+      rp += fred[ww][9];
+      rn += fred[ww][10];
+    }
+    const double sr = rn > 0.0 ? rs / rn : 0.0, pr = rn > 0.0 ? rp / rn : 0.0;
+    if (flags & F_RSPH) {
+      o.srs = sr;
+      o.prs = pr;
+    } else {
+      o.src = sr;
+      o.prc = pr;
+    }
+  }
   obs[r] = o;
   MomDev m = mom[r];
   m.n += 1.0;
@@ -659,6 +684,12 @@ __device__ __forceinline__ void finalize_compact(int r, const double *__restrict
   m.S2 += o.S * o.S;
   m.E += o.E;
   m.E2 += o.E * o.E;
+  if constexpr (RAD) {
+    m.srs += o.srs;
+    m.prs += o.prs;
+    m.src += o.src;
+    m.prc += o.prc;
+  }
   mom[r] = m;
 }
 
@@ -728,7 +759,7 @@ __device__ __forceinline__ bool metropolis_trial(const LatDev &L, const Site &s,
 // computes the pair's Philox draw itself and uses the half the pair's thread would give that position, so the
 // trials are the same and a wave's serial work halves (few waves per SIMD; LL lattices up to 80^3).
 template <int POT, bool MASK, bool SURF, bool FIELD, bool FLIP, bool FREE, bool MEAS, bool ROW, bool EMPTY = true,
-          bool SPLIT = false>
+          bool SPLIT = false, bool XR = false>
 // Launch shape (chosen by the autotuner): any multiple of 64 up to 1024 threads per workgroup; with xcd != 0
 // workgroup b runs logical tile (b & 7) * chunk + (b >> 3), so consecutive tiles share an XCD and its L2.
 // The sweep number is *sweep_base + k: graphs replay fixed arguments while the base advances on the device.
@@ -737,12 +768,17 @@ template <int POT, bool MASK, bool SURF, bool FIELD, bool FLIP, bool FREE, bool 
 // energy is known from its trial, so the lattice energy is the sum over colour-1 sites of twice their final site
 // energy minus their one-site terms, plus the one-site terms of colour 0. Each colour-0 site is counted (Q tensor,
 // one-site terms, count) by the colour-1 site on its +x side, or, at the last column of a free x axis, by the one on
-// its -x side. The workgroup writes kNQS FP64 partials to spart[q * nblk + workgroup].
+// its -x side. The workgroup writes kNQS FP64 partials to spart[q * nblk + workgroup] (kNQSR with XR, which adds the
+// radial order of the lattices that measure one). In an LL context with empty sites (stored as zero vectors) an
+// empty colour-1 position still counts the colour-0 sites it owns, and an empty colour-0 site is not counted.
 // ROW: every lattice of the context is row-aligned (gather_row): site decoding, the y / z neighbour indices and
 // the y / z free faces are scalar work.
 // A launch sweeps the workgroups [b0, b1) of the work list (one lattice group, or all of it); nblk is the length of
 // the whole list (the column stride of spart).
-__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(MEAS && (MASK || POT != MCL_POT_LL) ? 6 : 8, 8))) void k_sweep(
+// Register bound: 8 waves per SIMD (64 VGPRs); 6 for the measuring passes of masked, radial or non-LL contexts, 5 for
+// those that are masked and radial (at 6 they spilled 20-24 bytes per lane). No instance uses scratch.
+__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(
+    MEAS && MASK && XR ? 5 : (MEAS && (MASK || XR || POT != MCL_POT_LL) ? 6 : 8), 8))) void k_sweep(
     const LatDev *__restrict__ lats, CtrlDev *__restrict__ ctrl,
                                                 unsigned int *__restrict__ cnt, const int2 *__restrict__ blk,
                                                 int nblk, int b0, int b1, int chunk, int xcd, int colour,
@@ -760,7 +796,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(MEAS && (M
   // after them. nfin is the number of lattices of the launch when the context measures in the sweep, else 0.
   const int L0 = xcd ? (b & 7) * chunk + (b >> 3) : b;
   if (L0 < nfin) {
-    if (fin) finalize_compact(l0 + L0, spart, nblk, srange[l0 + L0], mom, obs);
+    if (fin) finalize_compact<!ROW>(l0 + L0, ROW ? 0 : lats[l0 + L0].flags, spart, nblk, srange[l0 + L0], mom, obs);
     return;
   }
   const int Lb = b0 + L0 - nfin;
@@ -793,9 +829,13 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(MEAS && (M
   } else if (sweep == ctrl[r].av_set_sweep) {
     av = ctrl[r].av_set;
   } else {
-    // accepted rotations of the previous sweep: every wave sums the 64 counters itself and applies the ang_var
-    // rule, so no wave waits at a barrier before issuing its loads (the counter load overlaps the gather)
-    unsigned a = cnt_r[((sweep - 1) % 3) * kCntSweep + lane * kCntStride];
+    // accepted rotations of the previous sweep: every wave sums the kCnt counters itself (one counter, one cache
+    // line, per lane) and applies the ang_var rule, so no wave waits at a barrier. Measured alternatives, all
+    // slower at 128^3: wave-uniform (scalar) loads of the counters, 16 %; the first wave of each workgroup loading
+    // them and passing ang_var through LDS after a workgroup barrier, 1.3 %; the sum and the rule after the first
+    // gather (the counter latency under the gather's), 4.5 %; a DPP sum instead of the LDS permutes, 0.6 %; 16 or 8
+    // counters, 1.4 or 2.8 % (the workgroups finish nearly together, and their atomics queue on fewer counters).
+    unsigned a = lane < kCnt ? cnt_r[((sweep - 1) % 3) * kCntSweep + lane * kCntStride] : 0u;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o, 64);
     av = ang_var_rule(ctrl[r].av_store[(sweep - 1) & 1], (unsigned)__builtin_amdgcn_readfirstlane(a), L.nvalid);
@@ -807,9 +847,11 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(MEAS && (M
     cnt_r[((sweep + 1) % 3) * kCntSweep + threadIdx.x * kCntStride] = 0u;
   }
   unsigned nacc = 0;
-  double qm[kNQS];
+  static_assert(!XR || MEAS, "XR is a measuring instance");
+  constexpr int NQS = XR ? kNQSR : kNQS;
+  double qm[NQS];
 #pragma unroll
-  for (int a = 0; a < kNQS; a++) qm[a] = 0.0;
+  for (int a = 0; a < NQS; a++) qm[a] = 0.0;
   auto add_q = [&](V3 v) {
     const double x = v.x, y = v.y, z = v.z;
     qm[1] += x * x;
@@ -820,6 +862,55 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(MEAS && (M
     qm[7] += z * z;
     qm[6] += 1.0;
   };
+  // radial order of a site at (i, j, k): c = n.r / r about the lattice's centre, sites within half a spacing of it
+  // (or of the axis) skipped; r in the xy plane for a cylinder (parameter_order.cpp:223-245, as k_measure)
+  auto add_r = [&](V3 v, int i, int j, int k) {
+    if constexpr (XR) {
+      if (radial_flags(L.flags)) {
+        const float rx = i - L.cx, ry = j - L.cy, rz = (L.flags & F_RSPH) ? k - L.cz : 0.0f;
+        const float rr = sqrtf(rx * rx + ry * ry + rz * rz);
+        if (rr >= 0.5f) {
+          const double c = (double)((v.x * rx + v.y * ry + v.z * rz) / rr);
+          qm[8] += 1.5 * c * c - 0.5;
+          qm[9] += c;
+          qm[10] += 1.0;
+        }
+      }
+    }
+  };
+  // an empty colour-1 position t of an LL context (its director is a zero vector, no trial) still counts the
+  // colour-0 sites it owns: its -x neighbour, and its +x neighbour at the end of a free x axis
+  auto own_from_empty = [&](uint32_t t) {
+    if constexpr (MEAS && MASK && POT == MCL_POT_LL) {
+      const uint32_t row = fdiv(t, L.mxh, L.sxh);
+      const int m = (int)(t - row * (uint32_t)L.nxh);
+      const int k = (int)fdiv(row, L.my, L.sy);
+      const int j = (int)row - k * L.ny;
+      const int i = 2 * m + ((j + k + 1) & 1);
+      if (i >= L.nx) return;  // padding position of an odd x axis
+      const uint32_t rowstart = L.off + row * (uint32_t)L.nxh;  // this row of the colour-0 block
+      double e1 = 0.0;
+      auto own0 = [&](int ii) {
+        const uint32_t q = rowstart + (uint32_t)(ii >> 1);
+        const int pt = ldx(PT, q);
+        if (pt == 0) return;
+        const V3 v{ldx(X, q), ldx(Y, q), ldx(Z, q)};
+        V3 sv{0.0f, 0.0f, 0.0f};
+        if constexpr (SURF) {
+          if (pt > 1) sv = V3{ldx(SX, q), ldx(SY, q), ldx(SZ, q)};
+        }
+        add_q(v);
+        if constexpr (SURF || FIELD) e1 += (double)site_terms<SURF, FIELD>(L, pt, sv, v, surf, elecE);
+        add_r(v, ii, j, k);
+      };
+      if (i > 0)
+        own0(i - 1);
+      else if (L.flags & F_PX)
+        own0(L.nx - 1);
+      if (!(L.flags & F_PX) && i + 1 == L.nx - 1) own0(i + 1);
+      qm[0] += e1;
+    }
+  };
 #pragma unroll
   for (int e = 0; e < (SPLIT ? 1 : SPT); e++) {
     Site s;
@@ -828,15 +919,22 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(MEAS && (M
       // first position of this lane group, wave-uniform (a wave covers 64 SPT consecutive positions)
       const uint32_t g = (uint32_t)bd.y + (uint32_t)(64 * SPT * __builtin_amdgcn_readfirstlane(threadIdx.x >> 6)) +
                          (uint32_t)(64 * e);
-      if (g >= (uint32_t)L.nc ||
-          !gather_row<MASK, SURF, FREE, MEAS && MASK, EMPTY && POT != MCL_POT_LL>(L, colour, g, lane, X, Y, Z, PT,
-                                                                                   SX, SY, SZ, s, self))
+      const bool in = g < (uint32_t)L.nc;
+      const bool ok = in && gather_row<MASK, SURF, FREE, MEAS && MASK, EMPTY && POT != MCL_POT_LL>(
+                                L, colour, g, lane, X, Y, Z, PT, SX, SY, SZ, s, self);
+      if (!ok) {
+        if (in) own_from_empty(g + (uint32_t)lane);
         continue;
+      }
     } else {
       const int t = SPLIT ? tw : tw + 64 * e;
-      if (t >= L.nc || !gather<MASK, SURF, FREE, MEAS && MASK, EMPTY && POT != MCL_POT_LL>(L, colour, t, X, Y, Z, PT,
-                                                                                          SX, SY, SZ, s, self))
+      const bool in = t < L.nc;
+      const bool ok = in && gather<MASK, SURF, FREE, MEAS && MASK, EMPTY && POT != MCL_POT_LL>(
+                                L, colour, t, X, Y, Z, PT, SX, SY, SZ, s, self);
+      if (!ok) {
+        if (in) own_from_empty((uint32_t)t);
         continue;
+      }
     }
     V3 nc;
     float E_fin;
@@ -856,34 +954,35 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(MEAS && (M
         if constexpr (SURF || FIELD) return (double)site_terms<SURF, FIELD>(L, pt, sv, v, surf, elecE);
         return 0.0;
       };
-      // the same for the x neighbour d of a masked lattice, whose point type and anchoring vector are loaded here
-      auto nb_terms = [&](int d) -> double {
+      double e2 = 2.0 * (double)E_fin;
+      add_q(nc);
+      add_r(nc, s.i, s.j, s.k);
+      e2 -= terms(s.ptv, s.sv, nc);
+      // a colour-0 site this site counts (x neighbour d, at x position ii): Q tensor, one-site terms, radial order.
+      // In a masked lattice its point type and anchoring vector are loaded here; an empty one (LL contexts) is not
+      // counted.
+      auto own = [&](int d, int ii) {
+        int pt = 1;
+        V3 sv{0.0f, 0.0f, 0.0f};
         if constexpr (MASK) {
           const uint32_t q = s.qx[d];
-          const int pt = ldx(PT, q);
-          V3 sv{0.0f, 0.0f, 0.0f};
+          pt = ldx(PT, q);
+          if constexpr (POT == MCL_POT_LL) {
+            if (pt == 0) return;
+          }
           if constexpr (SURF) {
             if (pt > 1) sv = V3{ldx(SX, q), ldx(SY, q), ldx(SZ, q)};
           }
-          return terms(pt, sv, s.nb[d]);
         }
-        return terms(1, V3{0.0f, 0.0f, 0.0f}, s.nb[d]);
+        add_q(s.nb[d]);
+        e2 += terms(pt, sv, s.nb[d]);
+        add_r(s.nb[d], ii, s.j, s.k);
       };
-      double e2 = 2.0 * (double)E_fin;
-      add_q(nc);
-      e2 -= terms(s.ptv, s.sv, nc);
       // the colour-0 sites this site counts: its -x neighbour, and its +x neighbour at the end of a free x axis
-      // (masked lattices take this path only when none of their sites is empty, so every colour-0 site has them)
       const bool own_m = FREE ? (s.ex & 2u) != 0 : true;
       const bool own_p = FREE && !(L.flags & F_PX) && s.i + 1 == L.nx - 1;
-      if (own_m) {
-        add_q(s.nb[1]);
-        e2 += nb_terms(1);
-      }
-      if (own_p) {
-        add_q(s.nb[0]);
-        e2 += nb_terms(0);
-      }
+      if (own_m) own(1, s.i > 0 ? s.i - 1 : L.nx - 1);
+      if (own_p) own(0, s.i + 1);
       qm[0] += e2;
     }
   }
@@ -901,7 +1000,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(MEAS && (M
     for (int w = 0; w < nw; w++) tot += wsum[w];
     if (tot) atomicAdd(&cnt_r[(sweep % 3) * kCntSweep + (lblk & (kCnt - 1)) * kCntStride], tot);
   }
-  if constexpr (MEAS && (MASK || POT != MCL_POT_LL)) {
+  if constexpr (MEAS && (MASK || XR || POT != MCL_POT_LL)) {
     static_assert(kNQS == 8, "row_rs8 reduces 8 quantities");
     // per wave and row of 16 lanes, the row sums of the 8 quantities (lanes 0-7 of each row write them). Only in
     // the instances that run at 6 waves per SIMD: under the 8-wave bound of the others the reduce-scatter spills.
@@ -910,12 +1009,26 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(MEAS && (M
     __shared__ double mred[16][4][kNQS];
     const double x = row_rs8(qm, lane);
     if ((lane & 8) == 0) mred[threadIdx.x >> 6][lane >> 4][rs8_index(lane)] = x;
+    // the radial sums (XR) by whole-wave sums
+    __shared__ double xred[XR ? 16 : 1][XR ? kNQSR - kNQS : 1];
+    if constexpr (XR) {
+#pragma unroll
+      for (int a = kNQS; a < NQS; a++) {
+        const double v = wave_sum(qm[a]);
+        if (lane == 0) xred[threadIdx.x >> 6][a - kNQS] = v;
+      }
+    }
     __syncthreads();
     if (threadIdx.x < kNQS) {
       double v = 0.0;
       const int nw = (int)(blockDim.x >> 6);
       for (int w = 0; w < nw; w++)
         v += (mred[w][0][threadIdx.x] + mred[w][1][threadIdx.x]) + (mred[w][2][threadIdx.x] + mred[w][3][threadIdx.x]);
+      spart[(size_t)threadIdx.x * nblk + Lb] = v;
+    } else if (XR && threadIdx.x < NQS) {
+      double v = 0.0;
+      const int nw = (int)(blockDim.x >> 6);
+      for (int w = 0; w < nw; w++) v += xred[w][threadIdx.x - kNQS];
       spart[(size_t)threadIdx.x * nblk + Lb] = v;
     }
   } else if constexpr (MEAS) {
@@ -989,7 +1102,7 @@ __global__ __launch_bounds__(kResBlock) void k_sweep_resident(
   float av_prev = ctrl[r].av_store[(sw0 - 1) & 1];
   unsigned c_prev = 0;
   if (sw0 != av_set_sweep) {
-    unsigned a = cnt_r[((sw0 - 1) % 3) * kCntSweep + lane * kCntStride];
+    unsigned a = lane < kCnt ? cnt_r[((sw0 - 1) % 3) * kCntSweep + lane * kCntStride] : 0u;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o, 64);
     c_prev = (unsigned)__builtin_amdgcn_readfirstlane(a);
@@ -1292,16 +1405,20 @@ __global__ __launch_bounds__(kFinBlock) void k_finalize(const LatDev *__restrict
 #pragma unroll
   for (int a = 0; a < kNQ; a++) v[a] = 0.0;
   if (ranges) {
-    // four partials per lane in flight, added in increasing order (the order finalize_compact keeps)
+    // four partials per lane in flight, added in increasing order (the order finalize_compact keeps); a lattice
+    // with a radial order has its sums in rows 8-10, added to the sphere (7-9) or cylinder (10-12) sums
     const int2 rg = ranges[r];
+    const bool rad = radial_flags(L.flags);
+    const int ro = (L.flags & F_RSPH) ? 7 : 10;
 #pragma unroll 1
     for (int p = rg.x + (int)threadIdx.x; p < rg.y; p += 4 * kFinBlock) {
-      double x[4][kNQS];
+      double x[4][kNQSR];
 #pragma unroll
       for (int u = 0; u < 4; u++) {
         const int q = p + u * kFinBlock;
 #pragma unroll
-        for (int a = 0; a < kNQS; a++) x[u][a] = q < rg.y ? partials[(size_t)a * nmblk + q] : 0.0;
+        for (int a = 0; a < kNQSR; a++)
+          x[u][a] = q < rg.y && (a < kNQS || rad) ? partials[(size_t)a * nmblk + q] : 0.0;
       }
 #pragma unroll
       for (int u = 0; u < 4; u++) {
@@ -1309,6 +1426,11 @@ __global__ __launch_bounds__(kFinBlock) void k_finalize(const LatDev *__restrict
 #pragma unroll
         for (int a = 0; a < 7; a++) v[a] += x[u][a];
         v[21] += x[u][7];
+        if (rad) {
+          v[ro] += x[u][8];
+          v[ro + 1] += x[u][9];
+          v[ro + 2] += x[u][10];
+        }
       }
     }
   } else {
@@ -1318,9 +1440,12 @@ __global__ __launch_bounds__(kFinBlock) void k_finalize(const LatDev *__restrict
         if (xq || basic_q(a)) v[a] += partials[(size_t)a * nmblk + p];
     }
   }
+  // quantities never written are left out of the reduction: the radial and pear ones without xq (the compact
+  // layout's radial rows come with xq), and the pear ones of the compact layout
+  auto used = [&](int a) { return (xq || basic_q(a)) && !(ranges && a >= 13 && a <= 20); };
 #pragma unroll
   for (int a = 0; a < kNQ; a++) {
-    if (!xq && !basic_q(a)) continue;  // quantities never written are left out of the reduction
+    if (!used(a)) continue;
     const double x = wave_sum(v[a]);
     if (lane == 0) red[w][a] = x;
   }
@@ -1329,7 +1454,7 @@ __global__ __launch_bounds__(kFinBlock) void k_finalize(const LatDev *__restrict
   double tot[kNQ];
   for (int a = 0; a < kNQ; a++) {
     double x = 0.0;
-    if (xq || basic_q(a))
+    if (used(a))
       for (int ww = 0; ww < kFinWaves; ww++) x += red[ww][a];
     tot[a] = x;
   }
@@ -1553,11 +1678,13 @@ struct Ctx {
   int4 *mblk_d = nullptr;
   int nmblk = 0;
   double *partials = nullptr;
-  // measurement folded into the colour-1 sweep pass (unmasked lattices without the radial or pear observables)
+  // measurement folded into the colour-1 sweep pass (every context but pear, GHRL with empty sites, lattices one
+  // site wide in x, and lattices measuring both radial orders); fr: it carries the radial order (kNQSR partials)
   bool fused = false;
+  bool fr = false;
   bool has_empty = false;  // some lattice has an empty site (masked GHRL / pear sweeps then check neighbours)
   bool split = false;      // one position per sweep thread (SPLIT): a small, not row-aligned context without flips
-  double *spart = nullptr;  // kNQS partials per workgroup of the sweep work list
+  double *spart = nullptr;  // kNQS (kNQSR with fr) partials per workgroup of the sweep work list
   size_t spart_cap = 0;     // workgroups spart holds
   int2 *sr_d = nullptr;     // per lattice: its workgroups [first, last) in the sweep work list
   MomDev *mom_d = nullptr;
@@ -1579,38 +1706,42 @@ struct Ctx {
 using SweepFn = void (*)(dim3, dim3, hipStream_t, const Ctx &, const LatGroup &, int, unsigned, bool, int);
 using MeasFn = void (*)(dim3, dim3, hipStream_t, const Ctx &, int);
 
-template <int POT, bool MASK, bool SURF, bool FIELD, bool FLIP, bool FREE, bool MEAS, bool ROW, bool SPLIT = false>
+template <int POT, bool MASK, bool SURF, bool FIELD, bool FLIP, bool FREE, bool MEAS, bool ROW, bool XR = false,
+          bool SPLIT = false>
 void launch_sweep_k(dim3 g, dim3 b, hipStream_t st, const Ctx &c, const LatGroup &gr, int colour, unsigned k,
                     int fin) {
   if constexpr (!FLIP && !ROW && !SPLIT) {
     if (c.split) {
-      launch_sweep_k<POT, MASK, SURF, FIELD, FLIP, FREE, MEAS, ROW, true>(g, b, st, c, gr, colour, k, fin);
+      launch_sweep_k<POT, MASK, SURF, FIELD, FLIP, FREE, MEAS, ROW, XR, true>(g, b, st, c, gr, colour, k, fin);
       return;
     }
   }
   const int nfin = c.fused ? gr.l1 - gr.l0 : 0;
   if constexpr (MASK && POT != MCL_POT_LL) {
     if (!c.has_empty) {
-      hipLaunchKernelGGL((k_sweep<POT, MASK, SURF, FIELD, FLIP, FREE, MEAS, ROW, false, SPLIT>), g, b, 0, st,
+      hipLaunchKernelGGL((k_sweep<POT, MASK, SURF, FIELD, FLIP, FREE, MEAS, ROW, false, SPLIT, XR>), g, b, 0, st,
                          c.lat_d, c.ctrl_d, c.cnt, c.blk_d, c.nblk, gr.b0, gr.b1, gr.chunk, c.xcd, colour,
                          c.sweep_base, k, c.X, c.Y, c.Z, c.PT, c.SX, c.SY, c.SZ, c.surf_d, c.spart, fin, gr.l0, nfin,
                          c.sr_d, c.mom_d, c.obs_d);
       return;
     }
   }
-  hipLaunchKernelGGL((k_sweep<POT, MASK, SURF, FIELD, FLIP, FREE, MEAS, ROW, true, SPLIT>), g, b, 0, st, c.lat_d,
+  hipLaunchKernelGGL((k_sweep<POT, MASK, SURF, FIELD, FLIP, FREE, MEAS, ROW, true, SPLIT, XR>), g, b, 0, st, c.lat_d,
                      c.ctrl_d, c.cnt, c.blk_d, c.nblk, gr.b0, gr.b1, gr.chunk, c.xcd, colour, c.sweep_base, k, c.X,
                      c.Y, c.Z, c.PT, c.SX, c.SY, c.SZ, c.surf_d, c.spart, fin, gr.l0, nfin, c.sr_d, c.mom_d, c.obs_d);
 }
 
 // meas (the measurement folded into the colour-1 pass) is never set for pear contexts: they always measure the pear
-// observables, which keep the separate measurement kernels, so no pear measuring instance is compiled.
+// observables, which keep the separate measurement kernels, so no pear measuring instance is compiled. A context
+// that measures a radial order takes the generic gather for its measuring passes (the same trials as the row path).
 template <int POT, bool MASK, bool SURF, bool FIELD, bool FLIP, bool FREE>
 void launch_sweep(dim3 g, dim3 b, hipStream_t st, const Ctx &c, const LatGroup &gr, int colour, unsigned k, bool meas,
                   int fin) {
   if constexpr (POT != MCL_POT_PEAR) {
     if (meas) {
-      if (c.row)
+      if (c.fr)
+        launch_sweep_k<POT, MASK, SURF, FIELD, FLIP, FREE, true, false, true>(g, b, st, c, gr, colour, k, fin);
+      else if (c.row)
         launch_sweep_k<POT, MASK, SURF, FIELD, FLIP, FREE, true, true>(g, b, st, c, gr, colour, k, fin);
       else
         launch_sweep_k<POT, MASK, SURF, FIELD, FLIP, FREE, true, false>(g, b, st, c, gr, colour, k, fin);
@@ -1855,7 +1986,7 @@ int build_worklist(Ctx *c, int bs, int xcd) {
       if (c->spart) MCL_HIP(hipFree(c->spart));
       c->spart = nullptr;
       c->spart_cap = 0;
-      if ((st = dev_alloc(&c->spart, (size_t)kNQS * blk.size()))) return st;
+      if ((st = dev_alloc(&c->spart, (size_t)(c->fr ? kNQSR : kNQS) * blk.size()))) return st;
       c->spart_cap = blk.size();
     }
     if (!c->sr_d && (st = dev_alloc(&c->sr_d, (size_t)c->nlat))) return st;
@@ -2078,9 +2209,15 @@ int create_ctx(const mcl_lattice_desc *lat, int nlat, Ctx *c) {
   MCL_HIP(hipMemcpyAsync(c->lat_d, c->ld.data(), sizeof(LatDev) * nlat, hipMemcpyHostToDevice, c->stream));
   const unsigned long long zero_base = 0ull;
   MCL_HIP(hipMemcpyAsync(c->sweep_base, &zero_base, sizeof(zero_base), hipMemcpyHostToDevice, c->stream));
-  // the measurement is folded into the sweep unless radial or pear observables are measured, some lattice is a
-  // single column in x, or some lattice has empty sites (each colour-0 site is counted by an x neighbour)
-  c->fused = !c->xq;
+  // the measurement is folded into the sweep unless the pear observables are measured, some lattice measures both
+  // radial orders, some lattice is a single column in x, or some lattice of a GHRL context has empty sites (each
+  // colour-0 site is counted by an x neighbour; an empty one of an LL context still counts them)
+  c->fused = c->pot != MCL_POT_PEAR;
+  bool any_radial = false;
+  for (int r = 0; r < nlat; r++) {
+    if ((c->ld[r].flags & F_RSPH) && (c->ld[r].flags & F_RCYL)) c->fused = false;
+    if (radial_flags(c->ld[r].flags)) any_radial = true;
+  }
   c->row = true;
   c->zero_empty = c->pot == MCL_POT_LL && c->mask;
   c->resident_ok = true;
@@ -2090,11 +2227,13 @@ int create_ctx(const mcl_lattice_desc *lat, int nlat, Ctx *c) {
     if (c->ld[r].nc > kResBlock) c->resident_one = false;
     if (c->ld[r].nx < 2) c->fused = false;
     if ((long long)c->ld[r].nvalid != (long long)c->ld[r].nx * c->ld[r].ny * c->ld[r].nz) {
-      c->fused = false;
+      if (!c->zero_empty) c->fused = false;
       c->has_empty = true;
     }
     if (c->ld[r].nx % 128 != 0) c->row = false;
   }
+  c->fr = c->fused && any_radial;
+  if (c->fr) c->row = false;  // the radial order is finalized by the generic instances only
   {
     long long npos = 0;
     for (int r = 0; r < nlat; r++) npos += c->ld[r].nc;
@@ -2270,8 +2409,8 @@ void mcl_destroy(void *ctx) { destroy_ctx(as_ctx(ctx)); }
 // Queues sweeps [k0, k0 + n) relative to the device sweep base (no timing, no advance).
 // The finalize of a measurement taken by the colour-1 sweep pass.
 static int run_fused_finalize(Ctx *c, const LatGroup &gr, unsigned advance) {
-  hipLaunchKernelGGL(k_finalize, dim3(gr.l1 - gr.l0), dim3(kFinBlock), 0, c->stream, c->lat_d, c->spart, c->nblk, 0,
-                     c->mom_d, c->obs_d, 1, (const int2 *)c->sr_d, gr.l0, c->sweep_base, advance);
+  hipLaunchKernelGGL(k_finalize, dim3(gr.l1 - gr.l0), dim3(kFinBlock), 0, c->stream, c->lat_d, c->spart, c->nblk,
+                     c->fr ? 1 : 0, c->mom_d, c->obs_d, 1, (const int2 *)c->sr_d, gr.l0, c->sweep_base, advance);
   MCL_HIP(hipGetLastError());
   return MCL_OK;
 }
@@ -2899,6 +3038,13 @@ int mcl_resident(void *ctx, int32_t *active) {
   Ctx *c = as_ctx(ctx);
   if (!c || !active) return fail(MCL_ERR_ARG, "null argument");
   *active = c->resident_ok && c->resident_on ? 1 : 0;
+  return MCL_OK;
+}
+
+int mcl_measure_in_sweep(void *ctx, int32_t *active) {
+  Ctx *c = as_ctx(ctx);
+  if (!c || !active) return fail(MCL_ERR_ARG, "null argument");
+  *active = c->fused ? 1 : 0;
   return MCL_OK;
 }
 

@@ -2,7 +2,9 @@
 
 The RNG counter is (site, sweep) and the acceptance count is an exact integer sum, so a lattice's trajectory does
 not depend on the launch shape, on graph replay or on the other lattices of its context: all of these must give
-bit-identical directors and moments.
+bit-identical directors, ang_var and counts. The FP64 moments of a context that measures in the sweep (these do)
+come from partial sums that follow the sweep's workgroups, finalized either in the next pass or by a trailing
+kernel, so their last bits may depend on the launch shape and on graph replay: they must agree to 1e-12.
 """
 import numpy as np
 import pytest
@@ -20,11 +22,16 @@ def state(eng, n):
            [eng.ang_var(r) for r in range(n)], [eng.sweep_stats(r) for r in range(n)]
 
 
+def close_moments(ma, mb):
+    return all(a["n"] == b["n"] and all(b[k] == pytest.approx(a[k], rel=1e-12, abs=1e-15) for k in a if k != "n")
+               for a, b in zip(ma, mb))
+
+
 def same(a, b):
     dirs_a, mom_a, av_a, st_a = a
     dirs_b, mom_b, av_b, st_b = b
-    return all(np.array_equal(x, y) for x, y in zip(dirs_a, dirs_b)) and mom_a == mom_b and av_a == av_b and \
-        st_a == st_b
+    return all(np.array_equal(x, y) for x, y in zip(dirs_a, dirs_b)) and close_moments(mom_a, mom_b) and \
+        av_a == av_b and st_a == st_b
 
 
 def run(mcl, shape=None, graph=0, calls=((30, 5),)):

@@ -3,7 +3,8 @@
 Empty sites are skipped by the sweep and are never neighbours. At a very high temperature nearly every trial is
 accepted: after a few sweeps every non-empty site must have moved, every empty site must still hold its input
 director bit for bit, and the acceptance count of a sweep must lie just below the number of non-empty sites. The
-trajectory does not depend on the launch shape nor on the other lattices of the context.
+trajectory does not depend on the launch shape nor on the other lattices of the context (the FP64 moments, whose
+partial sums follow the sweep's workgroups when the measurement is folded into the sweep, agree to 1e-12).
 """
 import numpy as np
 import pytest
@@ -54,7 +55,9 @@ def test_list_sweep_is_launch_shape_and_batch_independent(mcl):
     for shape, extra in (((64, 0), False), ((1024, 1), True), ((128, 0), True)):
         d, m = run(shape, extra)
         assert np.array_equal(d, ref_d), (shape, extra)
-        assert m == ref_m
+        assert m["n"] == ref_m["n"]
+        for k in ref_m:
+            assert m[k] == pytest.approx(ref_m[k], rel=1e-12, abs=1e-15), (shape, extra, k)
 
 
 def test_set_directors_mid_run_keeps_the_empty_sites(mcl):

@@ -1,8 +1,9 @@
 """Lattice groups (mcl_set_lattice_groups): a context larger than the Infinity Cache is swept group by group, each
 group through all the sweeps of a call before the next. A lattice's trajectory depends only on its own sites and
-sweep numbers, so the grouped order must give bit-identical directors, ang_var, acceptance counts and moments to
-the ungrouped one: eager and graph-replayed, with the separate measurement kernels (a batch with spheres) and with
-the measurement folded into the sweep (a bulk batch), under two launch shapes."""
+sweep numbers, so the grouped order must give bit-identical directors, ang_var and acceptance counts to the
+ungrouped one: eager and graph-replayed, a batch with spheres and a bulk batch (both measure in the sweep), under two
+launch shapes. The moments agree to 1e-12: a group's last measurement of a call is finalized by a trailing kernel,
+whose compiled arithmetic may round the last bit of an FP64 sum differently from the in-sweep finalize."""
 import numpy as np
 import pytest
 
@@ -42,7 +43,7 @@ def run(mcl, specs, group_bytes, graph, shape):
         return ngroups, out
 
 
-@pytest.mark.parametrize("specs", [MIXED, BULK, LARGE], ids=["mixed", "bulk-fused", "large"])
+@pytest.mark.parametrize("specs", [MIXED, BULK, LARGE], ids=["mixed", "bulk", "large"])
 @pytest.mark.parametrize("graph", [0, 16], ids=["eager", "graph"])
 @pytest.mark.parametrize("shape", [(256, 1), (512, 0)], ids=["256x", "512"])
 def test_groups_give_the_ungrouped_trajectories(mcl, specs, graph, shape):
@@ -54,7 +55,10 @@ def test_groups_give_the_ungrouped_trajectories(mcl, specs, graph, shape):
     for k, (a, b) in enumerate(zip(ref, got)):
         assert all(np.array_equal(x, y) for x, y in zip(a[0], b[0])), (k, "directors")
         assert a[1] == b[1] and a[2] == b[2], (k, "ang_var / counts")
-        assert a[3] == b[3], (k, "moments")
+        for ma, mb in zip(a[3], b[3]):
+            assert ma["n"] == mb["n"], (k, "moments")
+            for q in ma:
+                assert mb[q] == pytest.approx(ma[q], rel=1e-12, abs=1e-15), (k, "moments", q)
 
 
 def test_the_c4_batch_is_grouped_and_timed_in_whole_passes(mcl):
