@@ -833,8 +833,10 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(
     // line, per lane) and applies the ang_var rule, so no wave waits at a barrier. Measured alternatives, all
     // slower at 128^3: wave-uniform (scalar) loads of the counters, 16 %; the first wave of each workgroup loading
     // them and passing ang_var through LDS after a workgroup barrier, 1.3 %; the sum and the rule after the first
-    // gather (the counter latency under the gather's), 4.5 %; a DPP sum instead of the LDS permutes, 0.6 %; 16 or 8
-    // counters, 1.4 or 2.8 % (the workgroups finish nearly together, and their atomics queue on fewer counters).
+    // gather, 4.5 %, and with the counter load itself issued after the gather, 3 % (same register count); a DPP sum
+    // instead of the LDS permutes, 0.6 %; 16 or 8 counters, 1.4 or 2.8 % (the workgroups finish nearly together, and
+    // their atomics queue on fewer counters). Skipping the counters altogether (a frozen ang_var, for timing only)
+    // was 3-5 % faster: the colour-0 pass runs about 1 us longer than the colour-1 pass.
     unsigned a = lane < kCnt ? cnt_r[((sweep - 1) % 3) * kCntSweep + lane * kCntStride] : 0u;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o, 64);
