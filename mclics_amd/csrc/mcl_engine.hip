@@ -826,21 +826,25 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(
   float av;
   if (colour == 1) {
     av = ctrl[r].av_store[sweep & 1];
-  } else if (sweep == ctrl[r].av_set_sweep) {
-    av = ctrl[r].av_set;
   } else {
     // accepted rotations of the previous sweep: every wave sums the kCnt counters itself (one counter, one cache
-    // line, per lane) and applies the ang_var rule, so no wave waits at a barrier. Measured alternatives, all
-    // slower at 128^3: wave-uniform (scalar) loads of the counters, 16 %; the first wave of each workgroup loading
-    // them and passing ang_var through LDS after a workgroup barrier, 1.3 %; the sum and the rule after the first
-    // gather, 4.5 %, and with the counter load itself issued after the gather, 3 % (same register count); a DPP sum
-    // instead of the LDS permutes, 0.6 %; 16 or 8 counters, 1.4 or 2.8 % (the workgroups finish nearly together, and
-    // their atomics queue on fewer counters). Skipping the counters altogether (a frozen ang_var, for timing only)
-    // was 3-5 % faster: the colour-0 pass runs about 1 us longer than the colour-1 pass.
+    // line, per lane) and applies the ang_var rule, so no wave waits at a barrier. The counters are loaded and
+    // summed whether or not the host set ang_var for this sweep, so that their load is issued without waiting for
+    // the control block. Measured alternatives, all slower at 128^3: wave-uniform (scalar) loads of the counters,
+    // 16 %; the first wave of each workgroup loading them and passing ang_var through LDS after a workgroup barrier,
+    // 1.3 %; the sum and the rule after the first gather, 4.5 %, and with the counter load itself issued after the
+    // gather, 3 % (same register count); a DPP sum instead of the LDS permutes, 0.6 %; 16 or 8 counters, 1.4 or
+    // 2.8 % (the workgroups finish nearly together, and their atomics queue on fewer counters). Skipping the
+    // counters altogether (a frozen ang_var, for timing only) was 3-5 % faster. So were the work-list and sweep-base
+    // loads ahead of them: a standalone copy of the pass ran up to 6 % faster without them, but passing both as
+    // kernel arguments to eager launches of one-lattice contexts (a runtime switch, or an instance of its own) gave
+    // nothing in this kernel (-0.6 to -1.5 % at 128^3).
     unsigned a = lane < kCnt ? cnt_r[((sweep - 1) % 3) * kCntSweep + lane * kCntStride] : 0u;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o, 64);
-    av = ang_var_rule(ctrl[r].av_store[(sweep - 1) & 1], (unsigned)__builtin_amdgcn_readfirstlane(a), L.nvalid);
+    const float ruled =
+        ang_var_rule(ctrl[r].av_store[(sweep - 1) & 1], (unsigned)__builtin_amdgcn_readfirstlane(a), L.nvalid);
+    av = sweep == ctrl[r].av_set_sweep ? ctrl[r].av_set : ruled;
   }
   if (colour == 0 && lblk == 0 && threadIdx.x < kCnt) {
     // the lattice's first workgroup publishes ang_var for colour 1 and the next sweep, and clears the counters
