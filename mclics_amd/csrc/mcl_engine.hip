@@ -1408,6 +1408,9 @@ __global__ __launch_bounds__(kFinBlock) void k_finalize(const LatDev *__restrict
                                                      unsigned long long *__restrict__ sweep_base, unsigned advance) {
   const int r = l0 + (int)blockIdx.x;
   const LatDev &L = lats[r];
+  // the moments are loaded here, ahead of the partials and the barrier, so that the serial tail of thread 0 does not
+  // wait for them (this kernel ends a measured call)
+  const MomDev mprev = mom[r];
   __shared__ double red[kFinWaves][kNQ];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   double v[kNQ];
@@ -1492,7 +1495,7 @@ __global__ __launch_bounds__(kFinBlock) void k_finalize(const LatDev *__restrict
   o.divn = tot[20] * inv;
   obs[r] = o;
   if (accumulate) {
-    MomDev m = mom[r];
+    MomDev m = mprev;
     m.n += 1.0;
     m.S += o.S;
     m.S2 += o.S * o.S;
