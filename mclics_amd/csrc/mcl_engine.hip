@@ -1041,18 +1041,38 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(
       spart[(size_t)threadIdx.x * nblk + Lb] = v;
     }
   } else if constexpr (MEAS) {
-    __shared__ double mred[16][kNQS];
+    // unmasked LL contexts (8 waves per SIMD): every thread leaves its sums in the dynamic LDS, then wave w adds
+    // quantity w over the workgroup's threads in order (lane by lane, wave after wave) and sums its lanes: about 35
+    // VALU instructions per wave instead of kNQS wave sums (about 184 of the 615 at 128^3). Up to 512 threads all
+    // kNQS quantities go at once (32 KiB at 512 threads), above that half of them at a time (32 KiB at 1024).
+    extern __shared__ double mlds[];
+    const int nt = (int)blockDim.x, nw = nt >> 6, w = (int)__builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    auto reduce = [&](int a0, int na) {
+#pragma unroll 1
+      for (int a = w; a < na; a += nw) {
+        double v = 0.0;
+#pragma unroll 1
+        for (int j = 0; j < nw; j++) v += mlds[a * nt + 64 * j + lane];
+        v = wave_sum(v);
+        if (lane == 0) spart[(size_t)(a0 + a) * nblk + Lb] = v;
+      }
+    };
+    if (nt <= 512) {
 #pragma unroll
-    for (int a = 0; a < kNQS; a++) {
-      const double v = wave_sum(qm[a]);
-      if (lane == 0) mred[threadIdx.x >> 6][a] = v;
-    }
-    __syncthreads();
-    if (threadIdx.x < kNQS) {
-      double v = 0.0;
-      const int nw = (int)(blockDim.x >> 6);
-      for (int w = 0; w < nw; w++) v += mred[w][threadIdx.x];
-      spart[(size_t)threadIdx.x * nblk + Lb] = v;
+      for (int a = 0; a < kNQS; a++) mlds[a * nt + (int)threadIdx.x] = qm[a];
+      __syncthreads();
+      reduce(0, kNQS);
+    } else {
+      constexpr int H = kNQS / 2;
+#pragma unroll
+      for (int a = 0; a < H; a++) mlds[a * nt + (int)threadIdx.x] = qm[a];
+      __syncthreads();
+      reduce(0, H);
+      __syncthreads();
+#pragma unroll
+      for (int a = 0; a < H; a++) mlds[a * nt + (int)threadIdx.x] = qm[H + a];
+      __syncthreads();
+      reduce(H, H);
     }
   }
 }
@@ -1729,16 +1749,19 @@ void launch_sweep_k(dim3 g, dim3 b, hipStream_t st, const Ctx &c, const LatGroup
     }
   }
   const int nfin = c.fused ? gr.l1 - gr.l0 : 0;
+  // the measuring pass of an unmasked LL context reduces its partials through dynamic LDS (at most 32 KiB)
+  constexpr bool LDSRED = MEAS && !MASK && !XR && POT == MCL_POT_LL;
+  const unsigned shm = LDSRED ? (unsigned)(kNQS * sizeof(double)) * (b.x <= 512 ? b.x : b.x / 2) : 0u;
   if constexpr (MASK && POT != MCL_POT_LL) {
     if (!c.has_empty) {
-      hipLaunchKernelGGL((k_sweep<POT, MASK, SURF, FIELD, FLIP, FREE, MEAS, ROW, false, SPLIT, XR>), g, b, 0, st,
+      hipLaunchKernelGGL((k_sweep<POT, MASK, SURF, FIELD, FLIP, FREE, MEAS, ROW, false, SPLIT, XR>), g, b, shm, st,
                          c.lat_d, c.ctrl_d, c.cnt, c.blk_d, c.nblk, gr.b0, gr.b1, gr.chunk, c.xcd, colour,
                          c.sweep_base, k, c.X, c.Y, c.Z, c.PT, c.SX, c.SY, c.SZ, c.surf_d, c.spart, fin, gr.l0, nfin,
                          c.sr_d, c.mom_d, c.obs_d);
       return;
     }
   }
-  hipLaunchKernelGGL((k_sweep<POT, MASK, SURF, FIELD, FLIP, FREE, MEAS, ROW, true, SPLIT, XR>), g, b, 0, st, c.lat_d,
+  hipLaunchKernelGGL((k_sweep<POT, MASK, SURF, FIELD, FLIP, FREE, MEAS, ROW, true, SPLIT, XR>), g, b, shm, st, c.lat_d,
                      c.ctrl_d, c.cnt, c.blk_d, c.nblk, gr.b0, gr.b1, gr.chunk, c.xcd, colour, c.sweep_base, k, c.X,
                      c.Y, c.Z, c.PT, c.SX, c.SY, c.SZ, c.surf_d, c.spart, fin, gr.l0, nfin, c.sr_d, c.mom_d, c.obs_d);
 }
