@@ -1011,7 +1011,8 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(
   if constexpr (MEAS && (MASK || XR || POT != MCL_POT_LL)) {
     static_assert(kNQS == 8, "row_rs8 reduces 8 quantities");
     // per wave and row of 16 lanes, the row sums of the 8 quantities (lanes 0-7 of each row write them). Only in
-    // the instances that run at 6 waves per SIMD: under the 8-wave bound of the others the reduce-scatter spills.
+    // the instances that run at 6 or 5 waves per SIMD: under the 8-wave bound of the others the reduce-scatter
+    // spills, and they reduce through LDS (below).
     // The measuring passes that get here are those of GHRL contexts, of masked LL contexts (slabs, spheres, lattices
     // with empty sites) and of contexts that measure a radial order; pear contexts measure with the separate
     // kernels.
