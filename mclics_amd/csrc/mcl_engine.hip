@@ -823,35 +823,56 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(
                                           : (uint32_t)tw);
   const mcl::Philox4 R = mcl::philox4x32_10(ctr0, (uint32_t)sweep, (uint32_t)(sweep >> 32),
                                             kTagSweep + (uint32_t)colour, L.key0, L.key1);
-  float av;
+  // ang_var. Colour 1 reads what the lattice's first colour-0 workgroup published. In a colour-0 pass the accepted
+  // rotations of the previous sweep are summed from the kCnt counters (one counter, one cache line, per lane) and
+  // the rule applied; the counter load is issued before the check for an ang_var set by the host, so that it does
+  // not wait for the control block.
+  // W0 (unmasked contexts): only the first wave of each workgroup reads the counters; the other waves issue their
+  // first gather and take ang_var from LDS after a workgroup barrier placed behind it (take_av in the trial loop).
+  // 128^3 +3.3 % (8 times fewer requests to the 64 counter lines); masked contexts keep one read per wave (the slab
+  // was 2.5 % slower with the barrier). Measured alternatives, all slower at 128^3: wave-uniform (scalar) loads of
+  // the counters, 16 %; the first wave reading them and passing ang_var through LDS after a barrier before the
+  // gather, 1.3 %; every wave's sum and rule after its first gather, 4.5 %, and with the counter load itself issued
+  // after the gather, 3 %; a DPP sum instead of the LDS permutes, 0.6 %; 16 or 8 counters, 1.4 or 2.8 % (their
+  // atomics queue). Passing the work list and the sweep base as kernel arguments to eager launches of one-lattice
+  // contexts (a runtime switch, or an instance of its own) gave -0.6 to -1.5 % at 128^3.
+  constexpr bool W0 = !MASK;
+  float av = 0.0f;
+  __shared__ float s_av;
   if (colour == 1) {
     av = ctrl[r].av_store[sweep & 1];
-  } else {
-    // accepted rotations of the previous sweep: every wave sums the kCnt counters itself (one counter, one cache
-    // line, per lane) and applies the ang_var rule, so no wave waits at a barrier. The counters are loaded and
-    // summed whether or not the host set ang_var for this sweep, so that their load is issued without waiting for
-    // the control block. Measured alternatives, all slower at 128^3: wave-uniform (scalar) loads of the counters,
-    // 16 %; the first wave of each workgroup loading them and passing ang_var through LDS after a workgroup barrier,
-    // 1.3 %; the sum and the rule after the first gather, 4.5 %, and with the counter load itself issued after the
-    // gather, 3 % (same register count); a DPP sum instead of the LDS permutes, 0.6 %; 16 or 8 counters, 1.4 or
-    // 2.8 % (the workgroups finish nearly together, and their atomics queue on fewer counters). Skipping the
-    // counters altogether (a frozen ang_var, for timing only) was 3-5 % faster. So were the work-list and sweep-base
-    // loads ahead of them: a standalone copy of the pass ran up to 6 % faster without them, but passing both as
-    // kernel arguments to eager launches of one-lattice contexts (a runtime switch, or an instance of its own) gave
-    // nothing in this kernel (-0.6 to -1.5 % at 128^3).
+  } else if (!W0 || __builtin_amdgcn_readfirstlane(threadIdx.x >> 6) == 0) {
     unsigned a = lane < kCnt ? cnt_r[((sweep - 1) % 3) * kCntSweep + lane * kCntStride] : 0u;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o, 64);
     const float ruled =
         ang_var_rule(ctrl[r].av_store[(sweep - 1) & 1], (unsigned)__builtin_amdgcn_readfirstlane(a), L.nvalid);
-    av = sweep == ctrl[r].av_set_sweep ? ctrl[r].av_set : ruled;
+    const float avw = sweep == ctrl[r].av_set_sweep ? ctrl[r].av_set : ruled;
+    if constexpr (W0) {
+      if (lane == 0) s_av = avw;
+    } else {
+      av = avw;
+    }
   }
-  if (colour == 0 && lblk == 0 && threadIdx.x < kCnt) {
-    // the lattice's first workgroup publishes ang_var for colour 1 and the next sweep, and clears the counters
-    // of sweep + 1 (last read by the colour-0 pass of sweep - 1)
-    if (threadIdx.x == 0) ctrl[r].av_store[sweep & 1] = av;
-    cnt_r[((sweep + 1) % 3) * kCntSweep + threadIdx.x * kCntStride] = 0u;
-  }
+  // the lattice's first workgroup publishes ang_var for colour 1 and the next sweep, and clears the counters of
+  // sweep + 1 (last read by the colour-0 pass of sweep - 1)
+  auto publish = [&]() {
+    if (lblk == 0 && threadIdx.x < kCnt) {
+      if (threadIdx.x == 0) ctrl[r].av_store[sweep & 1] = av;
+      cnt_r[((sweep + 1) % 3) * kCntSweep + threadIdx.x * kCntStride] = 0u;
+    }
+  };
+  if (!W0 && colour == 0) publish();
+  // W0: after the first gather of every wave (all waves of the workgroup reach it)
+  auto take_av = [&]() {
+    if constexpr (W0) {
+      if (colour == 0) {
+        __syncthreads();
+        av = s_av;
+        publish();
+      }
+    }
+  };
   unsigned nacc = 0;
   static_assert(!XR || MEAS, "XR is a measuring instance");
   constexpr int NQS = XR ? kNQSR : kNQS;
@@ -928,6 +949,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(
       const bool in = g < (uint32_t)L.nc;
       const bool ok = in && gather_row<MASK, SURF, FREE, MEAS && MASK, EMPTY && POT != MCL_POT_LL>(
                                 L, colour, g, lane, X, Y, Z, PT, SX, SY, SZ, s, self);
+      if (e == 0) take_av();
       if (!ok) {
         if (in) own_from_empty(g + (uint32_t)lane);
         continue;
@@ -937,6 +959,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(
       const bool in = t < L.nc;
       const bool ok = in && gather<MASK, SURF, FREE, MEAS && MASK, EMPTY && POT != MCL_POT_LL>(
                                 L, colour, t, X, Y, Z, PT, SX, SY, SZ, s, self);
+      if (e == 0) take_av();
       if (!ok) {
         if (in) own_from_empty((uint32_t)t);
         continue;
