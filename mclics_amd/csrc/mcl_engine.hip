@@ -80,6 +80,7 @@ struct LatDev {
   uint32_t plane, ywrap, zwrap;  // nxh * ny, (ny - 1) * nxh, (nz - 1) * plane: colour-block strides
   int flags;
   int nvalid;
+  int kc;         // acceptance counters in use (a power of two up to kCnt; set at context creation)
   uint32_t off;   // pooled offset of the colour-0 block; colour 1 follows at off + nc (pool < 2^31 sites)
   uint32_t key0, key1;
   int surf0;      // first entry of this lattice in the pooled surface table
@@ -842,7 +843,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(
   if (colour == 1) {
     av = ctrl[r].av_store[sweep & 1];
   } else if (!W0 || __builtin_amdgcn_readfirstlane(threadIdx.x >> 6) == 0) {
-    unsigned a = lane < kCnt ? cnt_r[((sweep - 1) % 3) * kCntSweep + lane * kCntStride] : 0u;
+    unsigned a = lane < (MASK && !ROW ? L.kc : kCnt) ? cnt_r[((sweep - 1) % 3) * kCntSweep + lane * kCntStride] : 0u;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o, 64);
     const float ruled =
@@ -1029,7 +1030,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(
     unsigned tot = 0;
     const int nw = (int)(blockDim.x >> 6);
     for (int w = 0; w < nw; w++) tot += wsum[w];
-    if (tot) atomicAdd(&cnt_r[(sweep % 3) * kCntSweep + (lblk & (kCnt - 1)) * kCntStride], tot);
+    if (tot) atomicAdd(&cnt_r[(sweep % 3) * kCntSweep + (lblk & ((MASK && !ROW ? L.kc : kCnt) - 1)) * kCntStride], tot);
   }
   if constexpr (MEAS && (MASK || XR || POT != MCL_POT_LL)) {
     static_assert(kNQS == 8, "row_rs8 reduces 8 quantities");
@@ -2265,7 +2266,6 @@ int create_ctx(const mcl_lattice_desc *lat, int nlat, Ctx *c) {
   if (!surf_h.empty())
     MCL_HIP(hipMemcpyAsync(c->surf_d, surf_h.data(), sizeof(SurfDev) * surf_h.size(), hipMemcpyHostToDevice,
                            c->stream));
-  MCL_HIP(hipMemcpyAsync(c->lat_d, c->ld.data(), sizeof(LatDev) * nlat, hipMemcpyHostToDevice, c->stream));
   const unsigned long long zero_base = 0ull;
   MCL_HIP(hipMemcpyAsync(c->sweep_base, &zero_base, sizeof(zero_base), hipMemcpyHostToDevice, c->stream));
   // the measurement is folded into the sweep unless the pear observables are measured, some lattice measures both
@@ -2293,6 +2293,21 @@ int create_ctx(const mcl_lattice_desc *lat, int nlat, Ctx *c) {
   }
   c->fr = c->fused && any_radial;
   if (c->fr) c->row = false;  // the radial order is finalized by the generic instances only
+  // Acceptance counters in use per lattice (kc): a workgroup adds into counter (its index mod kc), a colour-0 pass
+  // sums kc of them, and the sums do not depend on kc. In masked contexts that are not row-aligned (the sphere
+  // batches) every colour-0 wave reads them, so a lattice uses about one per 8192 positions per colour (one for 16^3,
+  // 16 for 64^3, all kCnt from 128^3 on): the 512-lattice batch +1.9 %. Every other instance takes all kCnt as a
+  // constant: in unmasked contexts one wave per workgroup reads them (fewer counters were 0.4-2 % slower on single LL
+  // lattices, more workgroups per counter), and the run-time count cost the masked row-aligned slab 2 %.
+  for (LatDev &L : c->ld) {
+    L.kc = kCnt;
+    if (c->mask && !c->row) {
+      L.kc = 1;
+      while (L.kc < kCnt && (long long)L.kc * 8192 < (long long)L.nc) L.kc *= 2;
+    }
+  }
+  // the lattice records, complete now
+  MCL_HIP(hipMemcpyAsync(c->lat_d, c->ld.data(), sizeof(LatDev) * nlat, hipMemcpyHostToDevice, c->stream));
   {
     long long npos = 0;
     for (int r = 0; r < nlat; r++) npos += c->ld[r].nc;
