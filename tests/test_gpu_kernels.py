@@ -175,6 +175,36 @@ def test_ang_var_rule_on_device(mcl, orc):
                 av = eng.ang_var()
 
 
+@pytest.mark.parametrize("shape,holes", [((512, 0), False), ((64, 1), False), ((256, 1), True)],
+                         ids=["unmasked-512", "unmasked-64", "holes-256"])
+def test_ang_var_rule_two_pass(mcl, orc, shape, holes):
+    """The rule in the two-pass sweep, sweep by sweep against the oracle's rule, including a host-set value in the
+    middle of the run (evolve.cu:430-436): in an unmasked lattice only the first wave of each workgroup reads the
+    acceptance counters and hands ang_var to the others through LDS; a lattice with empty sites (masked, not
+    row-aligned) uses two counters at 32^3 and every colour-0 wave reads them."""
+    N = 32
+    pt = np.ones(N ** 3, dtype=np.int32)
+    if holes:
+        pt[np.random.default_rng(7).random(N ** 3) < 0.25] = 0
+    p = orc.params(N, N, N, T=1.1)
+    ni = orc.ic_random(pt, 5)
+    nvalid = int(pt.sum())
+    with mcl.Engine(lattice_from_orc(mcl, p, ni, pt, None, ang_var=0.5)) as eng:
+        eng.set_resident(False)
+        eng.set_launch(*shape)
+        av = 0.5
+        for s in range(12):
+            if s == 6:
+                eng.set_ang_var(0.9)
+                av = 0.9
+            eng.sweeps(1)
+            acc, done = eng.sweep_stats()
+            assert done == s + 1 and 0 < acc <= nvalid
+            want = orc.lib().orc_ang_var_update(av, acc, nvalid)
+            assert eng.ang_var() == np.float32(want), (s, acc)
+            av = eng.ang_var()
+
+
 def test_detailed_balance_independent_spins_gpu(mcl, orc):
     """Independent spins (LL with A = 0) in a dielectric field: the GPU chain samples exp(elecA cos^2 / T)."""
     N = 16
