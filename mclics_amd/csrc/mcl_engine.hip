@@ -825,9 +825,9 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(
   const mcl::Philox4 R = mcl::philox4x32_10(ctr0, (uint32_t)sweep, (uint32_t)(sweep >> 32),
                                             kTagSweep + (uint32_t)colour, L.key0, L.key1);
   // ang_var. Colour 1 reads what the lattice's first colour-0 workgroup published. In a colour-0 pass the accepted
-  // rotations of the previous sweep are summed from the kCnt counters (one counter, one cache line, per lane) and
-  // the rule applied; the counter load is issued before the check for an ang_var set by the host, so that it does
-  // not wait for the control block.
+  // rotations of the previous sweep are summed from the lattice's counters (kCnt of them, L.kc in masked contexts
+  // that are not row-aligned; one counter, one cache line, per lane) and the rule applied; the counter load is issued
+  // before the check for an ang_var set by the host, so that it does not wait for the control block.
   // W0 (unmasked contexts): only the first wave of each workgroup reads the counters; the other waves issue their
   // first gather and take ang_var from LDS after a workgroup barrier placed behind it (take_av in the trial loop).
   // 128^3 +3.3 % (8 times fewer requests to the 64 counter lines); masked contexts keep one read per wave (the slab
